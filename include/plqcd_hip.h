@@ -1,0 +1,214 @@
+/*
+ * plqcd_hip.h -- C ABI of the MI355X Wilson hopping-matrix library.
+ *
+ * Drop-in for PLQCD's hopping-matrix path.  Host C code that used
+ *   #include "plqcd.h"          (include/plqcd.h:17-18 -> start.h, dirac.h)
+ * includes this header instead and links libplqcd_hip.so.  The field types,
+ * the global geometry struct `plqcd_g`, the start.c geometry API and the
+ * utils.c helpers keep the reference's names, layouts and argument meanings;
+ * the hopping routines gain a `_hip` suffix.
+ *
+ * Differences from the reference, all deliberate:
+ *  - One process, one GPU: NPROC0..3 must be 1 (no MPI); face[] is 0.
+ *  - Errors never exit(): functions return an error code (int), a negative
+ *    time (double) or -1 index, and plqcd_hip_last_error() holds the message.
+ *  - The ISA-specific split-layout tables of plqcd_g (ipt_eo_yzt,
+ *    *_sse_split, *_avx_split, *_mic_split, phip512...) are not provided.
+ */
+#ifndef PLQCD_HIP_H
+#define PLQCD_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include <stdio.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+/* every declaration below is exported from libplqcd_hip.so */
+#pragma GCC visibility push(default)
+
+/* ------------------------------------------------------------------ types */
+/* Same memory layout as include/su3.h:52-75 (complex = two doubles, re, im). */
+#ifdef __cplusplus
+typedef struct { double re, im; } plqcd_dcomplex;
+#else
+typedef _Complex double plqcd_dcomplex;
+#endif
+
+/* su3.h:52-55: row-major 3x3, c<row><col> */
+typedef struct {
+   plqcd_dcomplex c00, c01, c02, c10, c11, c12, c20, c21, c22;
+} su3;
+/* su3.h:57-60 */
+typedef struct { plqcd_dcomplex c0, c1, c2; } su3_vector;
+/* su3.h:67-70: spin-major, s0..s3 each c0..c2 */
+typedef struct { su3_vector s0, s1, s2, s3; } spinor;
+/* su3.h:72-75 */
+typedef struct { su3_vector s0, s1; } halfspinor;
+
+/* ------------------------------------------------------ global parameters */
+/* Subset of struct _PLQCD_GLOBAL_PARAMS (include/plqcd_global_params.h:67-205)
+ * with the same field names and meanings. */
+struct _PLQCD_GLOBAL_PARAMS {
+   int nprocs[4];   /* processes per direction: always 1 here              */
+   int nthread;     /* NTHREAD from the input file (host threads)          */
+   int cpr[4];      /* coordinates of this process: 0                      */
+   int npr[8];      /* neighbour process ids: 0                            */
+   int LX, LY, LZ, LT; /* global lattice extents (x,y,z,t)                  */
+   int latdims[4];  /* local extents = global extents                      */
+   int VOLUME;      /* latdims[0]*latdims[1]*latdims[2]*latdims[3]         */
+   int face[4];     /* boundary sizes: 0 with one process                  */
+   int *ipt_eo;     /* lexicographic -> even-odd index                     */
+   int *ipt_lex;    /* even-odd -> lexicographic index                     */
+   int *nn_bnde[8]; /* boundary maps: NULL with one process                */
+   int *nn_bndo[8];
+   int **iup;       /* iup[i][mu]: even-odd index of the +mu neighbour      */
+   int **idn;       /* idn[i][mu]: even-odd index of the -mu neighbour      */
+   halfspinor *phip[4]; /* host scratch of the CPU code: NULL (unused)      */
+   halfspinor *phim[4];
+   int ALIGN;       /* host allocations are aligned to 2^ALIGN bytes        */
+   int init;        /* 1 once init_plqcd succeeded                          */
+};
+typedef struct _PLQCD_GLOBAL_PARAMS plqcd_global_params;
+extern struct _PLQCD_GLOBAL_PARAMS plqcd_g;
+
+/* ------------------------------------------- start.c (include/start.h) */
+/* start.h:30 / start.c:19-232.  argv[1] names a key/value input file with the
+ * keys L0..L3, NPROC0..3, NTHREAD, ALIGN (defaults 4,4,4,4 / 1 / 1 / 16).
+ * Returns 0 on success (also when already initialised), -1 when argv[1] is
+ * missing (as the reference), a positive code for an unreadable file or
+ * invalid parameters (the reference exits instead). */
+int init_plqcd(int argc, char *argv[]);
+/* Programmatic form of init_plqcd (no input file). */
+int plqcd_init_dims(int L0, int L1, int L2, int L3, int nthread, int align);
+/* start.h:37 / start.c:237-247: releases the geometry tables and device state. */
+void finalize_plqcd(void);
+/* start.h:44 / start.c:1169-1183 */
+int ipr(int n[]);
+/* start.h:50 / start.c:1190-1212 */
+void comp_cpr(int ip, int n[]);
+/* start.h:61 / start.c:1227-1248 */
+void set_npr(void);
+/* start.h:67 / start.c:1256-1270; leaves xcor untouched on a bad index */
+void lex2cart(int ix, int xcor[]);
+/* start.h:72 / start.c:1278-1298; returns -1 on out-of-range coordinates */
+int cart2lex(int xcor[]);
+/* start.h:76 / start.c:255-526 (even-odd order and periodic neighbours).
+ * Returns 0, or nonzero on invalid dimensions (the reference is void). */
+int set_geometry(void);
+/* start.h:84 / start.c:1308-1343; *pid = *xeo = -1 on bad coordinates */
+void find_site(int xg[], int *pid, int *xeo);
+/* start.h:88 / start.c:1351-1428: same text format */
+void print_params(FILE *fp);
+
+/* ---------------------------------------------- utils.c (include/utils.h) */
+void *alloc(size_t size, size_t alignment);   /* utils.h:13 / utils.c:23-33 */
+void *amalloc(size_t size, int p);            /* utils.h:31 / utils.c:35-65 */
+void afree(void *addr);                       /* utils.h:32 / utils.c:68-90 */
+double stop_watch(double t);                  /* utils.h:43 / utils.c:120-125 */
+
+/* ------------------------------------------------------------- errors */
+/* Message of the last failure in this thread ("" if none). */
+const char *plqcd_hip_last_error(void);
+
+/* ------------------------------------------- hopping matrix (dirac.h) */
+#define PLQCD_HOP_COMPAT 0  /* the reference's compiled output, see below  */
+#define PLQCD_HOP_WILSON 1  /* Eq. 2.1 of plqcd.pdf                        */
+
+/* Mode used by the three drop-in calls below.  PLQCD_HOP_COMPAT (default)
+ * reproduces lib/dirac.c's ASSYMBLY routines as compiled: the mu=0 backward
+ * lower spin components are assigned, not accumulated (dirac.c:373-374,
+ * :756-757), and the oe forward pass reads the links of site i-V/2
+ * (dirac.c:643-644).  PLQCD_HOP_WILSON computes Eq. 2.1 exactly.
+ * Returns 0, or -1 for an unknown mode. */
+int plqcd_hip_set_mode(int mode);
+
+/* Replaces plqcd_hopping_matrix_eo_sse3_assymbly (include/dirac.h:21,
+ * lib/dirac.c:35-408): qout[0,V/2) = H_eo qin[V/2,V).  Host pointers in the
+ * reference layout, even-odd ordered, after init_plqcd.  Returns the wall
+ * seconds of the call including host<->device transfers, or a negative value
+ * on failure.  qin and qout may alias (qin's odd half is read, qout's even
+ * half written). */
+double plqcd_hopping_matrix_eo_hip(spinor *qin, spinor *qout, su3 *u);
+/* Replaces plqcd_hopping_matrix_oe_sse3_assymbly (include/dirac.h:22,
+ * lib/dirac.c:414-791): qout[V/2,V) = H_oe qin[0,V/2). */
+double plqcd_hopping_matrix_oe_hip(spinor *qin, spinor *qout, su3 *u);
+/* eo followed by oe (tests/test_hopping.c:296-297) in one launch. */
+double plqcd_hopping_matrix_hip(spinor *qin, spinor *qout, su3 *u);
+/* When enabled, the drop-in calls skip the gauge upload if `u` is the same
+ * pointer as in the previous call (the caller promises the links did not
+ * change).  Disabled by default.  plqcd_hip_gauge_changed() forces the next
+ * call to upload. */
+void plqcd_hip_cache_gauge(int enable);
+void plqcd_hip_gauge_changed(void);
+
+/* ------------------------------------------------- device-resident API */
+/* Storage precisions and link formats of device fields. */
+#define PLQCD_PREC_F64 0
+#define PLQCD_PREC_F32 1
+#define PLQCD_PREC_F16 2  /* spinors: fp16 with a float scale per site      */
+#define PLQCD_LINKS_18 18 /* full 3x3 complex                               */
+#define PLQCD_LINKS_12 12 /* rows 0,1; row 2 = conj(row0 x row1) (SU(3) only) */
+
+/* Output parity of a hopping application. */
+#define PLQCD_PARITY_EVEN 0 /* eo: even outputs from odd inputs              */
+#define PLQCD_PARITY_ODD 1  /* oe: odd outputs from even inputs              */
+#define PLQCD_PARITY_ALL 2  /* full H                                        */
+
+typedef struct plqcd_hip_lattice plqcd_hip_lattice;
+typedef struct plqcd_hip_gauge plqcd_hip_gauge;
+typedef struct plqcd_hip_field plqcd_hip_field;
+
+/* A lattice with extents L[0..3] (x,y,z,t), each even and >= 2, on the
+ * current HIP device, with its own stream.  All calls taking a lattice
+ * object are asynchronous on that stream unless stated. */
+int plqcd_hip_lattice_create(const int L[4], plqcd_hip_lattice **out);
+void plqcd_hip_lattice_destroy(plqcd_hip_lattice *lat);
+/* Use an existing hipStream_t (NULL: back to the lattice's own stream). */
+int plqcd_hip_lattice_set_stream(plqcd_hip_lattice *lat, void *hip_stream);
+void *plqcd_hip_lattice_stream(plqcd_hip_lattice *lat);
+int plqcd_hip_lattice_sync(plqcd_hip_lattice *lat);
+/* Event timer on the lattice stream: start, then stop returns milliseconds
+ * between the two events (waits for the stop event). */
+int plqcd_hip_timer_start(plqcd_hip_lattice *lat);
+int plqcd_hip_timer_stop(plqcd_hip_lattice *lat, double *ms);
+
+int plqcd_hip_gauge_create(plqcd_hip_lattice *lat, int prec, int links, plqcd_hip_gauge **out);
+void plqcd_hip_gauge_destroy(plqcd_hip_gauge *g);
+/* u: 4V links in the reference layout u[4*i+mu], even-odd site order.
+ * Synchronous with respect to the host buffer. */
+int plqcd_hip_gauge_upload(plqcd_hip_gauge *g, const su3 *u);
+/* Device bytes of the gauge field. */
+size_t plqcd_hip_gauge_bytes(const plqcd_hip_gauge *g);
+
+int plqcd_hip_field_create(plqcd_hip_lattice *lat, int prec, int nrhs, plqcd_hip_field **out);
+void plqcd_hip_field_destroy(plqcd_hip_field *f);
+/* V spinors in the reference layout, even-odd order, for right-hand side
+ * `rhs`.  Synchronous with respect to the host buffer. */
+int plqcd_hip_field_upload(plqcd_hip_field *f, int rhs, const spinor *host);
+int plqcd_hip_field_download(const plqcd_hip_field *f, int rhs, spinor *host);
+size_t plqcd_hip_field_bytes(const plqcd_hip_field *f);
+
+/* out = H in on the given output parity (the other parity of `out` is not
+ * touched), mode PLQCD_HOP_COMPAT or PLQCD_HOP_WILSON.  in and out must be
+ * distinct fields of the gauge's lattice and of the same precision and RHS
+ * count.  Asynchronous. */
+int plqcd_hip_hop(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip_field *out,
+                  int parity, int mode);
+
+/* ------------------------------------------------ synthetic inputs */
+/* Seeded, counter-based and thread-order independent: the value at each
+ * (site, component) depends only on (seed, even-odd site index, component).
+ * Random SU(3): rows 0,1 of a 3x3 matrix of N(0,1)+iN(0,1) entries are
+ * Gram-Schmidt orthonormalised, row 2 = conj(row0 x row1).  V = volume. */
+void plqcd_random_gauge_su3(su3 *u, int V, uint64_t seed);
+void plqcd_random_spinor_gauss(spinor *s, int V, uint64_t seed);
+/* u = 1 on every link (free field). */
+void plqcd_unit_gauge(su3 *u, int V);
+
+#pragma GCC visibility pop
+#ifdef __cplusplus
+}
+#endif
+#endif /* PLQCD_HIP_H */

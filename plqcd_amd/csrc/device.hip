@@ -1,0 +1,374 @@
+// Device objects behind the C-ABI handles (lattice, gauge, spinor fields) and
+// the kernels that repack the reference's host layout (array of structures in
+// even-odd site order, include/su3.h:52-75) into the structure-of-arrays HBM
+// layouts described in device_common.h, converting precision on the way.
+#include <algorithm>
+#include <cstring>
+#include <new>
+
+#include "fields.h"
+
+namespace {
+
+constexpr size_t kStageBytes = size_t(64) << 20;  // staging chunk for host transfers
+constexpr int kPackBlock = 256;
+
+// --------------------------------------------------------- spinor repacking
+
+// aos: n sites starting at even-odd index i0, 12 complex doubles each.
+template <int PREC>
+__global__ void pack_spinor(const d2 *__restrict__ aos, SpinorIO<PREC> f, int i0, int n)
+{
+   const int t = blockIdx.x * blockDim.x + threadIdx.x;
+   if (t >= n) return;
+   const int i = i0 + t;
+   const int p = i >= f.Vh;
+   const int j = i - p * f.Vh;
+   typename SpinorIO<PREC>::C s[12];
+#pragma unroll
+   for (int c = 0; c < 12; c++) {
+      const d2 v = aos[(size_t)t * 12 + c];
+      s[c].x = v.x;
+      s[c].y = v.y;
+   }
+   f.store(p, j, s);
+}
+
+template <int PREC>
+__global__ void unpack_spinor(d2 *__restrict__ aos, SpinorIO<PREC> f, int i0, int n)
+{
+   const int t = blockIdx.x * blockDim.x + threadIdx.x;
+   if (t >= n) return;
+   const int i = i0 + t;
+   const int p = i >= f.Vh;
+   const int j = i - p * f.Vh;
+   typename SpinorIO<PREC>::C s[12];
+   f.load(p, j, s);
+#pragma unroll
+   for (int c = 0; c < 12; c++) aos[(size_t)t * 12 + c] = d2{(double)s[c].x, (double)s[c].y};
+}
+
+// ----------------------------------------------------------- link repacking
+
+// aos: n links starting at link index l0 = 4*i0 (u[4*i+mu]), 9 complex each.
+__global__ void pack_gauge(const d2 *__restrict__ aos, void *dst, int prec, int links, int Vh, int l0, int n)
+{
+   const int t = blockIdx.x * blockDim.x + threadIdx.x;
+   if (t >= n) return;
+   const int l = l0 + t;
+   const int i = l >> 2, mu = l & 3;
+   const int p = i >= Vh;
+   const int j = i - p * Vh;
+   const d2 *u = aos + (size_t)t * 9;
+   const size_t lk = (size_t)(p * 4 + mu);  // link slot [parity][mu]
+   if (prec == PLQCD_PREC_F64) {
+      const int nk = links / 2;
+      d2 *o = static_cast<d2 *>(dst) + lk * nk * Vh + j;
+      for (int k = 0; k < nk; k++) o[(size_t)k * Vh] = u[k];
+   } else if (prec == PLQCD_PREC_F32 && links == 18) {
+      f2 *o = static_cast<f2 *>(dst) + lk * 9 * Vh + j;
+      for (int k = 0; k < 9; k++) o[(size_t)k * Vh] = f2{(float)u[k].x, (float)u[k].y};
+   } else if (prec == PLQCD_PREC_F32) {
+      f4 *o = static_cast<f4 *>(dst) + lk * 3 * Vh + j;
+      for (int k = 0; k < 3; k++)
+         o[(size_t)k * Vh] = f4{(float)u[2 * k].x, (float)u[2 * k].y, (float)u[2 * k + 1].x, (float)u[2 * k + 1].y};
+   } else {
+      u4 *a = static_cast<u4 *>(dst) + lk * Vh + j;
+      u2 *b = reinterpret_cast<u2 *>(static_cast<u4 *>(dst) + (size_t)8 * Vh) + lk * Vh + j;
+      u4 wa;
+      wa.x = f2_to_h2((float)u[0].x, (float)u[0].y);
+      wa.y = f2_to_h2((float)u[1].x, (float)u[1].y);
+      wa.z = f2_to_h2((float)u[2].x, (float)u[2].y);
+      wa.w = f2_to_h2((float)u[3].x, (float)u[3].y);
+      u2 wb;
+      wb.x = f2_to_h2((float)u[4].x, (float)u[4].y);
+      wb.y = f2_to_h2((float)u[5].x, (float)u[5].y);
+      *a = wa;
+      *b = wb;
+   }
+}
+
+template <int PREC>
+SpinorIO<PREC> spinor_io(const plqcd_hip_field *f, int rhs)
+{
+   SpinorIO<PREC> s;
+   s.base = reinterpret_cast<decltype(s.base)>(static_cast<char *>(f->data) + (size_t)rhs * f->rhs_bytes);
+   s.Vh = f->lat->Vh;
+   return s;
+}
+
+int ensure_stage(plqcd_hip_lattice *lat)
+{
+   if (lat->stage) return 0;
+   PLQCD_HIP_TRY(hipMalloc(&lat->stage, kStageBytes));
+   lat->stage_bytes = kStageBytes;
+   return 0;
+}
+
+int check_rhs(const plqcd_hip_field *f, int rhs)
+{
+   if (!f) return plqcd::set_error(1, "null field");
+   if (rhs < 0 || rhs >= f->nrhs) return plqcd::set_error(1, "rhs %d out of range [0,%d)", rhs, f->nrhs);
+   return 0;
+}
+
+}  // namespace
+
+// ================================================================== lattice
+
+extern "C" int plqcd_hip_lattice_create(const int L[4], plqcd_hip_lattice **out)
+{
+   plqcd::clear_error();
+   if (!L || !out) return plqcd::set_error(1, "plqcd_hip_lattice_create: null argument");
+   *out = nullptr;
+   long long V = 1;
+   for (int mu = 0; mu < 4; mu++) {
+      if (L[mu] < 2 || (L[mu] & 1))
+         return plqcd::set_error(2, "plqcd_hip_lattice_create: L[%d] = %d must be even and >= 2", mu, L[mu]);
+      V *= L[mu];
+   }
+   if (V > (1LL << 30)) return plqcd::set_error(2, "plqcd_hip_lattice_create: volume %lld too large", V);
+   plqcd_hip_lattice *lat = new (std::nothrow) plqcd_hip_lattice();
+   if (!lat) return plqcd::set_error(4, "out of host memory");
+   for (int mu = 0; mu < 4; mu++) lat->L[mu] = L[mu];
+   lat->V = (int)V;
+   lat->Vh = (int)(V / 2);
+   lat->Lxh = L[0] / 2;
+   hipError_t e = hipGetDevice(&lat->device);
+   if (e == hipSuccess) e = hipStreamCreateWithFlags(&lat->own_stream, hipStreamNonBlocking);
+   if (e == hipSuccess) e = hipEventCreate(&lat->ev_start);
+   if (e == hipSuccess) e = hipEventCreate(&lat->ev_stop);
+   if (e != hipSuccess) {
+      delete lat;
+      return plqcd::set_error(10, "plqcd_hip_lattice_create: %s", hipGetErrorString(e));
+   }
+   lat->stream = lat->own_stream;
+   *out = lat;
+   return 0;
+}
+
+extern "C" void plqcd_hip_lattice_destroy(plqcd_hip_lattice *lat)
+{
+   if (!lat) return;
+   (void)hipStreamSynchronize(lat->stream);
+   if (lat->stage) (void)hipFree(lat->stage);
+   (void)hipEventDestroy(lat->ev_start);
+   (void)hipEventDestroy(lat->ev_stop);
+   (void)hipStreamDestroy(lat->own_stream);
+   delete lat;
+}
+
+extern "C" int plqcd_hip_lattice_set_stream(plqcd_hip_lattice *lat, void *stream)
+{
+   if (!lat) return plqcd::set_error(1, "null lattice");
+   lat->stream = stream ? static_cast<hipStream_t>(stream) : lat->own_stream;
+   return 0;
+}
+
+extern "C" void *plqcd_hip_lattice_stream(plqcd_hip_lattice *lat) { return lat ? (void *)lat->stream : nullptr; }
+
+extern "C" int plqcd_hip_lattice_sync(plqcd_hip_lattice *lat)
+{
+   if (!lat) return plqcd::set_error(1, "null lattice");
+   PLQCD_HIP_TRY(hipStreamSynchronize(lat->stream));
+   return 0;
+}
+
+extern "C" int plqcd_hip_timer_start(plqcd_hip_lattice *lat)
+{
+   if (!lat) return plqcd::set_error(1, "null lattice");
+   PLQCD_HIP_TRY(hipEventRecord(lat->ev_start, lat->stream));
+   return 0;
+}
+
+extern "C" int plqcd_hip_timer_stop(plqcd_hip_lattice *lat, double *ms)
+{
+   if (!lat || !ms) return plqcd::set_error(1, "null argument");
+   PLQCD_HIP_TRY(hipEventRecord(lat->ev_stop, lat->stream));
+   PLQCD_HIP_TRY(hipEventSynchronize(lat->ev_stop));
+   float f = 0.f;
+   PLQCD_HIP_TRY(hipEventElapsedTime(&f, lat->ev_start, lat->ev_stop));
+   *ms = f;
+   return 0;
+}
+
+// ==================================================================== gauge
+
+extern "C" int plqcd_hip_gauge_create(plqcd_hip_lattice *lat, int prec, int links, plqcd_hip_gauge **out)
+{
+   plqcd::clear_error();
+   if (!lat || !out) return plqcd::set_error(1, "plqcd_hip_gauge_create: null argument");
+   *out = nullptr;
+   const bool ok = (prec == PLQCD_PREC_F64 && (links == 18 || links == 12)) ||
+                   (prec == PLQCD_PREC_F32 && (links == 18 || links == 12)) || (prec == PLQCD_PREC_F16 && links == 12);
+   if (!ok) return plqcd::set_error(2, "plqcd_hip_gauge_create: unsupported precision %d with %d-real links", prec, links);
+   plqcd_hip_gauge *g = new (std::nothrow) plqcd_hip_gauge();
+   if (!g) return plqcd::set_error(4, "out of host memory");
+   g->lat = lat;
+   g->prec = prec;
+   g->links = links;
+   g->bytes = plqcd::gauge_bytes(prec, links, lat->Vh);
+   hipError_t e = hipMalloc(&g->data, g->bytes);
+   if (e != hipSuccess) {
+      delete g;
+      return plqcd::set_error(5, "plqcd_hip_gauge_create: hipMalloc(%zu): %s", plqcd::gauge_bytes(prec, links, lat->Vh),
+                              hipGetErrorString(e));
+   }
+   *out = g;
+   return 0;
+}
+
+extern "C" void plqcd_hip_gauge_destroy(plqcd_hip_gauge *g)
+{
+   if (!g) return;
+   (void)hipStreamSynchronize(g->lat->stream);
+   (void)hipFree(g->data);
+   delete g;
+}
+
+extern "C" size_t plqcd_hip_gauge_bytes(const plqcd_hip_gauge *g) { return g ? g->bytes : 0; }
+
+extern "C" int plqcd_hip_gauge_upload(plqcd_hip_gauge *g, const su3 *u)
+{
+   if (!g || !u) return plqcd::set_error(1, "plqcd_hip_gauge_upload: null argument");
+   plqcd_hip_lattice *lat = g->lat;
+   if (int rc = ensure_stage(lat)) return rc;
+   const int nlinks = 4 * lat->V;
+   const int per = (int)(lat->stage_bytes / sizeof(su3));
+   for (int l0 = 0; l0 < nlinks; l0 += per) {
+      const int n = std::min(per, nlinks - l0);
+      PLQCD_HIP_TRY(hipMemcpyAsync(lat->stage, u + l0, (size_t)n * sizeof(su3), hipMemcpyHostToDevice, lat->stream));
+      hipLaunchKernelGGL(pack_gauge, dim3((n + kPackBlock - 1) / kPackBlock), dim3(kPackBlock), 0, lat->stream,
+                         static_cast<const d2 *>(lat->stage), g->data, g->prec, g->links, lat->Vh, l0, n);
+      PLQCD_HIP_TRY(hipGetLastError());
+   }
+   PLQCD_HIP_TRY(hipStreamSynchronize(lat->stream));
+   return 0;
+}
+
+// =================================================================== fields
+
+extern "C" int plqcd_hip_field_create(plqcd_hip_lattice *lat, int prec, int nrhs, plqcd_hip_field **out)
+{
+   plqcd::clear_error();
+   if (!lat || !out) return plqcd::set_error(1, "plqcd_hip_field_create: null argument");
+   *out = nullptr;
+   if (prec < PLQCD_PREC_F64 || prec > PLQCD_PREC_F16)
+      return plqcd::set_error(2, "plqcd_hip_field_create: bad precision %d", prec);
+   if (nrhs < 1 || nrhs > 4096) return plqcd::set_error(2, "plqcd_hip_field_create: bad RHS count %d", nrhs);
+   plqcd_hip_field *f = new (std::nothrow) plqcd_hip_field();
+   if (!f) return plqcd::set_error(4, "out of host memory");
+   f->lat = lat;
+   f->prec = prec;
+   f->nrhs = nrhs;
+   f->rhs_bytes = (2 * plqcd::spinor_parity_bytes(prec, lat->Vh) + 255) & ~size_t(255);
+   f->bytes = f->rhs_bytes * nrhs;
+   hipError_t e = hipMalloc(&f->data, f->bytes);
+   if (e != hipSuccess) {
+      delete f;
+      return plqcd::set_error(5, "plqcd_hip_field_create: hipMalloc: %s", hipGetErrorString(e));
+   }
+   *out = f;
+   return 0;
+}
+
+extern "C" void plqcd_hip_field_destroy(plqcd_hip_field *f)
+{
+   if (!f) return;
+   (void)hipStreamSynchronize(f->lat->stream);
+   (void)hipFree(f->data);
+   delete f;
+}
+
+extern "C" size_t plqcd_hip_field_bytes(const plqcd_hip_field *f) { return f ? f->bytes : 0; }
+
+int plqcd::field_upload_range(plqcd_hip_field *f, int rhs, const spinor *host, int first, int last)
+{
+   if (int rc = check_rhs(f, rhs)) return rc;
+   if (!host) return plqcd::set_error(1, "plqcd_hip_field_upload: null host pointer");
+   plqcd_hip_lattice *lat = f->lat;
+   if (first < 0 || last > lat->V || first > last) return plqcd::set_error(1, "bad site range");
+   if (int rc = ensure_stage(lat)) return rc;
+   const int per = (int)(lat->stage_bytes / sizeof(spinor));
+   for (int i0 = first; i0 < last; i0 += per) {
+      const int n = std::min(per, last - i0);
+      PLQCD_HIP_TRY(
+          hipMemcpyAsync(lat->stage, host + i0, (size_t)n * sizeof(spinor), hipMemcpyHostToDevice, lat->stream));
+      const dim3 grid((n + kPackBlock - 1) / kPackBlock), block(kPackBlock);
+      const d2 *src = static_cast<const d2 *>(lat->stage);
+      switch (f->prec) {
+         case PLQCD_PREC_F64:
+            hipLaunchKernelGGL(pack_spinor<PLQCD_PREC_F64>, grid, block, 0, lat->stream, src,
+                               spinor_io<PLQCD_PREC_F64>(f, rhs), i0, n);
+            break;
+         case PLQCD_PREC_F32:
+            hipLaunchKernelGGL(pack_spinor<PLQCD_PREC_F32>, grid, block, 0, lat->stream, src,
+                               spinor_io<PLQCD_PREC_F32>(f, rhs), i0, n);
+            break;
+         default:
+            hipLaunchKernelGGL(pack_spinor<PLQCD_PREC_F16>, grid, block, 0, lat->stream, src,
+                               spinor_io<PLQCD_PREC_F16>(f, rhs), i0, n);
+      }
+      PLQCD_HIP_TRY(hipGetLastError());
+   }
+   PLQCD_HIP_TRY(hipStreamSynchronize(lat->stream));
+   return 0;
+}
+
+int plqcd::field_download_range(const plqcd_hip_field *f, int rhs, spinor *host, int first, int last)
+{
+   if (int rc = check_rhs(f, rhs)) return rc;
+   if (!host) return plqcd::set_error(1, "plqcd_hip_field_download: null host pointer");
+   plqcd_hip_lattice *lat = f->lat;
+   if (first < 0 || last > lat->V || first > last) return plqcd::set_error(1, "bad site range");
+   if (int rc = ensure_stage(lat)) return rc;
+   const int per = (int)(lat->stage_bytes / sizeof(spinor));
+   for (int i0 = first; i0 < last; i0 += per) {
+      const int n = std::min(per, last - i0);
+      const dim3 grid((n + kPackBlock - 1) / kPackBlock), block(kPackBlock);
+      d2 *dst = static_cast<d2 *>(lat->stage);
+      switch (f->prec) {
+         case PLQCD_PREC_F64:
+            hipLaunchKernelGGL(unpack_spinor<PLQCD_PREC_F64>, grid, block, 0, lat->stream, dst,
+                               spinor_io<PLQCD_PREC_F64>(f, rhs), i0, n);
+            break;
+         case PLQCD_PREC_F32:
+            hipLaunchKernelGGL(unpack_spinor<PLQCD_PREC_F32>, grid, block, 0, lat->stream, dst,
+                               spinor_io<PLQCD_PREC_F32>(f, rhs), i0, n);
+            break;
+         default:
+            hipLaunchKernelGGL(unpack_spinor<PLQCD_PREC_F16>, grid, block, 0, lat->stream, dst,
+                               spinor_io<PLQCD_PREC_F16>(f, rhs), i0, n);
+      }
+      PLQCD_HIP_TRY(hipGetLastError());
+      PLQCD_HIP_TRY(
+          hipMemcpyAsync(host + i0, lat->stage, (size_t)n * sizeof(spinor), hipMemcpyDeviceToHost, lat->stream));
+   }
+   PLQCD_HIP_TRY(hipStreamSynchronize(lat->stream));
+   return 0;
+}
+
+extern "C" int plqcd_hip_field_upload(plqcd_hip_field *f, int rhs, const spinor *host)
+{
+   if (!f) return plqcd::set_error(1, "null field");
+   return plqcd::field_upload_range(f, rhs, host, 0, f->lat->V);
+}
+
+extern "C" int plqcd_hip_field_download(const plqcd_hip_field *f, int rhs, spinor *host)
+{
+   if (!f) return plqcd::set_error(1, "null field");
+   return plqcd::field_download_range(f, rhs, host, 0, f->lat->V);
+}
+
+// Copies the sites [first, last) of one RHS between two fields of the same
+// lattice and precision (device to device, asynchronous).
+int plqcd::field_copy_range(const plqcd_hip_field *src, plqcd_hip_field *dst, int rhs, int parity)
+{
+   if (src->prec != dst->prec || src->lat != dst->lat) return plqcd::set_error(1, "field copy: mismatch");
+   const size_t pb = plqcd::spinor_parity_bytes(src->prec, src->lat->Vh);
+   if (src->prec == PLQCD_PREC_F16) return plqcd::set_error(1, "field copy: F16 parity copy not supported");
+   const char *s = static_cast<const char *>(src->data) + rhs * src->rhs_bytes + parity * pb;
+   char *d = static_cast<char *>(dst->data) + rhs * dst->rhs_bytes + parity * pb;
+   PLQCD_HIP_TRY(hipMemcpyAsync(d, s, pb, hipMemcpyDeviceToDevice, src->lat->stream));
+   return 0;
+}

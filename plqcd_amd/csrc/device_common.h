@@ -1,0 +1,173 @@
+// Device-side definitions shared by the HIP translation units: the objects
+// behind the opaque C-ABI handles, the HBM layouts of fields, and complex
+// helpers on ext_vector types (so that vector loads, stores and non-temporal
+// builtins apply directly).
+//
+// HBM layouts (Vh = V/2 sites per parity; j = index of a site inside its
+// parity, i.e. even-odd index minus parity*Vh, which keeps the reference's
+// site order):
+//   spinor F64  [rhs][parity][12 colour-spin][Vh]  double2   (192 B/site)
+//   spinor F32  [rhs][parity][6 pairs][Vh]        float4    ( 96 B/site)
+//   spinor F16  [rhs][parity][3 quads][Vh]        8 x fp16  ( 48 B/site)
+//               + [rhs][parity][Vh] float max-abs scale     (  4 B/site)
+//   gauge  F64  [parity][mu][9 or 6][Vh]          double2
+//   gauge  F32  [parity][mu][9][Vh] float2  or  [parity][mu][3][Vh] float4
+//   gauge  F16  [parity][mu][Vh] 8 x fp16 (row 0 + row1.c0) and
+//               [parity][mu][Vh] 4 x fp16 (row1.c1, row1.c2)
+// Component order inside a spinor: c = 3*spin + colour.  Inside a link:
+// k = 3*row + col.  With 12-real links row 2 is rebuilt as conj(row0 x row1).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <cstddef>
+#include <cstdint>
+
+#include "internal.h"
+
+typedef double d2 __attribute__((ext_vector_type(2)));
+typedef float f2 __attribute__((ext_vector_type(2)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u2 __attribute__((ext_vector_type(2)));
+
+struct plqcd_hip_lattice {
+   int L[4];
+   int V, Vh, Lxh;
+   int device;
+   hipStream_t own_stream;
+   hipStream_t stream;
+   hipEvent_t ev_start, ev_stop;
+   void *stage;  // device staging buffer for host<->device repacking
+   size_t stage_bytes;
+};
+
+struct plqcd_hip_gauge {
+   plqcd_hip_lattice *lat;
+   int prec;   // PLQCD_PREC_*
+   int links;  // 18 or 12
+   void *data;
+   size_t bytes;
+};
+
+struct plqcd_hip_field {
+   plqcd_hip_lattice *lat;
+   int prec;
+   int nrhs;
+   void *data;
+   size_t rhs_bytes;  // bytes per right-hand side (both parities)
+   size_t bytes;
+};
+
+namespace plqcd {
+
+// Bytes of one parity of one RHS of a spinor field (data and, for F16, scales).
+inline size_t spinor_parity_bytes(int prec, int Vh)
+{
+   switch (prec) {
+      case PLQCD_PREC_F64: return size_t(Vh) * 12 * 16;
+      case PLQCD_PREC_F32: return size_t(Vh) * 6 * 16;
+      default: return size_t(Vh) * (3 * 16 + 4);
+   }
+}
+inline size_t gauge_bytes(int prec, int links, int Vh)
+{
+   const size_t n = size_t(2) * 4 * Vh;  // links
+   switch (prec) {
+      case PLQCD_PREC_F64: return n * links / 2 * 16;
+      case PLQCD_PREC_F32: return n * links * 4;
+      default: return n * (16 + 8);
+   }
+}
+
+#define PLQCD_HIP_TRY(expr)                                                                            \
+   do {                                                                                                \
+      hipError_t e_ = (expr);                                                                          \
+      if (e_ != hipSuccess)                                                                            \
+         return ::plqcd::set_error(10, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, \
+                                   __LINE__);                                                          \
+   } while (0)
+
+}  // namespace plqcd
+
+// ------------------------------------------------------------- complex math
+
+template <typename R>
+struct cvec;
+template <>
+struct cvec<double> {
+   typedef d2 t;
+};
+template <>
+struct cvec<float> {
+   typedef f2 t;
+};
+
+// c + a*b
+template <typename C>
+__device__ __forceinline__ C cmadd(C a, C b, C c)
+{
+   C r;
+   r.x = fma(a.x, b.x, c.x);
+   r.x = fma(-a.y, b.y, r.x);
+   r.y = fma(a.x, b.y, c.y);
+   r.y = fma(a.y, b.x, r.y);
+   return r;
+}
+// c + conj(a)*b
+template <typename C>
+__device__ __forceinline__ C cjmadd(C a, C b, C c)
+{
+   C r;
+   r.x = fma(a.x, b.x, c.x);
+   r.x = fma(a.y, b.y, r.x);
+   r.y = fma(a.x, b.y, c.y);
+   r.y = fma(-a.y, b.x, r.y);
+   return r;
+}
+template <typename C>
+__device__ __forceinline__ C cmul(C a, C b)
+{
+   C r;
+   r.x = a.x * b.x - a.y * b.y;
+   r.y = a.x * b.y + a.y * b.x;
+   return r;
+}
+// i*a
+template <typename C>
+__device__ __forceinline__ C ci(C a)
+{
+   C r;
+   r.x = -a.y;
+   r.y = a.x;
+   return r;
+}
+template <typename C>
+__device__ __forceinline__ C cconj(C a)
+{
+   C r;
+   r.x = a.x;
+   r.y = -a.y;
+   return r;
+}
+
+// Row 2 of an SU(3) matrix from rows 0 and 1: u2_a = conj(u0_b u1_c - u0_c u1_b).
+template <typename C>
+__device__ __forceinline__ void reconstruct_row2(C u[9])
+{
+   u[6] = cconj(cmul(u[1], u[5]) - cmul(u[2], u[4]));
+   u[7] = cconj(cmul(u[2], u[3]) - cmul(u[0], u[5]));
+   u[8] = cconj(cmul(u[0], u[4]) - cmul(u[1], u[3]));
+}
+
+__device__ __forceinline__ f2 h2_to_f2(uint32_t w)
+{
+   __half2 h = *reinterpret_cast<__half2 *>(&w);
+   float2 f = __half22float2(h);
+   return f2{f.x, f.y};
+}
+__device__ __forceinline__ uint32_t f2_to_h2(float a, float b)
+{
+   __half2 h = __floats2half2_rn(a, b);
+   return *reinterpret_cast<uint32_t *>(&h);
+}

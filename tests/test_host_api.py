@@ -1,0 +1,150 @@
+"""Host-side checks of libplqcd_hip.so that need no GPU: exported symbols,
+the lib/start.c geometry API against the reference's own print_params dumps,
+error behaviour, aligned allocation and the seeded input generator."""
+import ctypes as C
+import gzip
+import hashlib
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, ROOT, load_golden
+
+
+def test_library_exports_every_declared_symbol(hp):
+    from plqcd_amd import capi
+    lib = capi.lib()
+    names = capi.header_functions()
+    assert len(names) >= 40
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    for v in capi.header_variables():
+        C.c_int.in_dll(lib, v)  # raises if absent
+    # every declared function has a ctypes prototype in the binding
+    assert set(names) <= set(capi._PROTOS), set(names) - set(capi._PROTOS)
+
+
+@pytest.mark.parametrize("L", [(4, 4, 4, 4), (8, 4, 6, 4)])
+def test_print_params_matches_reference_dump(hp, tmp_path, L):
+    inp = tmp_path / "in.txt"
+    inp.write_text(f"L0 {L[0]}\nL1 {L[1]}\nL2 {L[2]}\nL3 {L[3]}\n"
+                   "NPROC0 1\nNPROC1 1\nNPROC2 1\nNPROC3 1\nNTHREAD 1\nALIGN 6\n")
+    hp.finalize_plqcd()
+    assert hp.init_plqcd(str(inp)) == 0
+    out = tmp_path / "geo.txt"
+    hp.print_params(str(out))
+    ref = gzip.decompress((GOLDEN / ("geometry_%dx%dx%dx%d.txt.gz" % L)).read_bytes())
+    assert out.read_bytes() == ref
+    hp.finalize_plqcd()
+
+
+def test_geometry_tables_and_helpers(hp, orc):
+    L = (6, 4, 4, 8)
+    hp.init_dims(L)
+    g = hp.geometry()
+    ipt_eo, ipt_lex, iup, idn = orc.geometry(L)
+    assert g.VOLUME == 768 and g.latdims == L
+    assert np.array_equal(g.ipt_eo, ipt_eo) and np.array_equal(g.ipt_lex, ipt_lex)
+    assert np.array_equal(g.iup, iup) and np.array_equal(g.idn, idn)
+    for lex in (0, 1, 17, 767):
+        xc = hp.lex2cart(lex)
+        assert hp.cart2lex(xc) == lex
+        pid, xeo = hp.find_site(xc)
+        assert pid == 0 and xeo == ipt_eo[lex]
+    # out-of-range inputs return error values instead of exiting
+    assert hp.cart2lex((6, 0, 0, 0)) == -1
+    assert hp.find_site((0, 0, 0, 8)) == (-1, -1)
+    assert hp.lex2cart(768) == (-1, -1, -1, -1)
+    hp.finalize_plqcd()
+
+
+def test_init_errors_do_not_exit(hp, tmp_path):
+    hp.finalize_plqcd()
+    assert hp.init_plqcd(str(tmp_path / "missing.input")) != 0
+    bad = tmp_path / "bad.input"
+    bad.write_text("L0 5\nL1 4\nL2 4\nL3 4\n")
+    assert hp.init_plqcd(str(bad)) != 0
+    bad.write_text("L0 4\nL1 4\nL2 4\nL3 4\nNPROC3 2\n")
+    assert hp.init_plqcd(str(bad)) != 0
+    assert hp.plqcd_g().init == 0
+    from plqcd_amd import capi
+    assert "NPROC3" in capi.last_error()
+    argv = (C.c_char_p * 2)(b"x", None)
+    assert capi.lib().init_plqcd(1, argv) == -1
+
+
+def test_hopping_before_init_fails_cleanly(hp):
+    hp.finalize_plqcd()
+    from plqcd_amd import capi
+    buf = np.zeros(16)
+    assert capi.lib().plqcd_hopping_matrix_eo_hip(buf.ctypes.data, buf.ctypes.data, buf.ctypes.data) < 0
+    assert "init_plqcd" in capi.last_error()
+    assert capi.lib().plqcd_hip_set_mode(7) != 0
+
+
+def test_amalloc_alignment(hp):
+    from plqcd_amd import capi
+    lib = capi.lib()
+    ptrs = []
+    for p in (4, 6, 12, 16):
+        a = lib.amalloc(1000, p)
+        assert a and a % (1 << p) == 0
+        ptrs.append(a)
+    assert lib.amalloc(0, 4) is None
+    for a in ptrs:
+        lib.afree(a)
+    lib.afree(12345)  # unknown address: ignored
+    t0 = lib.stop_watch(0.0)
+    assert lib.stop_watch(t0) >= 0.0
+
+
+def test_generator_pinned_by_fixtures(hp):
+    for name in ("seed_4x4x4x4", "seed_8x4x6x4", "seed_8x8x8x16"):
+        g = load_golden(f"{name}.npz")
+        V = int(np.prod(g["L"]))
+        u = hp.random_gauge_su3(V, int(g["gauge_seed"]))
+        q = hp.random_spinor(V, int(g["spinor_seed"]))
+        assert hashlib.sha256(u.tobytes()).hexdigest() == str(g["sha_gauge"])
+        assert hashlib.sha256(q.tobytes()).hexdigest() == str(g["sha_spinor"])
+
+
+def test_generator_su3_and_gauss(hp):
+    V = 4096
+    u = hp.random_gauge_su3(V, 3).reshape(-1, 3, 3)
+    eye = np.eye(3)
+    assert np.abs(u @ u.conj().transpose(0, 2, 1) - eye).max() < 1e-14
+    assert np.abs(np.linalg.det(u) - 1).max() < 1e-14
+    q = hp.random_spinor(V, 4).view(np.float64).ravel()
+    assert abs(q.mean()) < 0.02 and abs(q.std() - 1) < 0.02
+    # counter-based: same seed same field, different seed different field
+    assert np.array_equal(hp.random_spinor(64, 4), hp.random_spinor(V, 4)[:64])
+    assert not np.allclose(hp.random_spinor(64, 5), hp.random_spinor(64, 4))
+
+
+def test_c_host_program_compiles_against_header(tmp_path):
+    """A C host program written against include/plqcd_hip.h links and resolves."""
+    src = tmp_path / "drop_in.c"
+    src.write_text(r'''
+#include "plqcd_hip.h"
+#include <stdlib.h>
+int main(void) {
+    if (plqcd_init_dims(4, 4, 4, 4, 1, 6)) return 1;
+    spinor *pin = amalloc(plqcd_g.VOLUME * sizeof(spinor), plqcd_g.ALIGN);
+    su3 *u = amalloc(4 * plqcd_g.VOLUME * sizeof(su3), plqcd_g.ALIGN);
+    if (!pin || !u) return 2;
+    plqcd_random_spinor_gauss(pin, plqcd_g.VOLUME, 1);
+    plqcd_random_gauge_su3(u, plqcd_g.VOLUME, 2);
+    int x[4]; lex2cart(5, x);
+    int ok = cart2lex(x) == 5 && plqcd_g.iup[0][0] == 128;
+    afree(pin); afree(u); finalize_plqcd();
+    return ok ? 0 : 3;
+}
+''')
+    exe = tmp_path / "drop_in"
+    lib_dir = ROOT / "plqcd_amd"
+    subprocess.run(["gcc", "-std=c11", "-Wall", "-Werror", f"-I{ROOT / 'include'}", str(src), "-o", str(exe),
+                    f"-L{lib_dir}", "-lplqcd_hip", f"-Wl,-rpath,{lib_dir}"], check=True)
+    r = subprocess.run([str(exe)], capture_output=True, timeout=60)
+    assert r.returncode == 0, r
