@@ -197,6 +197,59 @@ size_t plqcd_hip_field_bytes(const plqcd_hip_field *f);
 int plqcd_hip_hop(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip_field *out,
                   int parity, int mode);
 
+/* ------------------------------------------------------------- solvers */
+/* The reference declares solver parameters but ships none
+ * (include/plqcd_global_params.h:197-201); these are new, built on the
+ * hopping matrix in wilson mode with M = 1 - kappa H, M^+ = gamma5 M gamma5. */
+
+/* out = M in (dagger = 0) or M^+ in (dagger = 1), kappa_rhs[r] for RHS r
+ * (host array of in->nrhs values).  Synchronous. */
+int plqcd_hip_apply_m(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip_field *out,
+                      const double *kappa_rhs, int dagger);
+
+typedef struct {
+   double kappa;            /* M = 1 - kappa H                                  */
+   double tol;              /* stop at ||b - M^+M x|| / ||b|| <= tol (FP64)     */
+   int max_iter;            /* iterations of the inner CG                       */
+   double delta;            /* reliable update when the iterated residual drops
+                               by delta since the last one; <= 0: 1e-3 for F32,
+                               0.1 for F16 sloppy, only at convergence for F64  */
+   int use_graph;           /* replay one captured hipGraph per iteration       */
+   const double *kappa_rhs; /* optional per-RHS kappa (b->nrhs values)          */
+   const double *tol_rhs;   /* optional per-RHS tolerance                       */
+} plqcd_cg_params;
+
+typedef struct {
+   int iterations;        /* inner CG iterations of this RHS                    */
+   int reliable_updates;  /* FP64 residual recomputations                       */
+   int converged;         /* 1 when true_residual <= tol                        */
+   double true_residual;  /* ||b - M^+M x|| / ||b|| computed in FP64            */
+   double seconds;        /* wall time of the whole solve call                  */
+   double hop_applications; /* applications of H spent on this RHS              */
+} plqcd_cg_stats;
+
+/* Solves M^+M x = b for every RHS of b (at most 32) with CG on the normal
+ * equations.  g: F64 gauge (18- or 12-real links) used for the true residual;
+ * g_sloppy: gauge of the inner iteration, any precision (NULL or g: plain FP64
+ * CG).  x (F64, same RHS count) is overwritten, starting from 0.  stats:
+ * b->nrhs entries.  Returns 0 when the solve ran (check stats[r].converged). */
+int plqcd_hip_cg_solve(const plqcd_hip_gauge *g, const plqcd_hip_gauge *g_sloppy, const plqcd_hip_field *b,
+                       plqcd_hip_field *x, const plqcd_cg_params *params, plqcd_cg_stats *stats);
+
+/* Launch shape of the hopping kernels for storage precision `prec` (-1: all
+ * precisions), process-wide; the autotuner sets it.
+ *   order 0: block b -> tile b; 1: each XCD takes a contiguous eighth of the
+ *   lattice; 2: XCDs sweep the lattice together, XCD k taking runs of `group`
+ *   consecutive tiles; block: threads per block (64, 128, 256 or 512);
+ *   tmarch > 0: full-lattice wilson applications whose time extent is a
+ *   multiple of tmarch use the t-march kernel (one thread walks tmarch
+ *   t-slices of a column), otherwise the one-site-per-thread gather kernel.
+ * Results do not depend on it beyond rounding.  Returns 0, or nonzero for
+ * bad values. */
+int plqcd_hip_hop_tuning(int prec, int order, int group, int block, int tmarch);
+/* Current launch shape of precision `prec`: out = {order, group, block, tmarch}. */
+int plqcd_hip_hop_get_tuning(int prec, int out[4]);
+
 /* ------------------------------------------------ synthetic inputs */
 /* Seeded, counter-based and thread-order independent: the value at each
  * (site, component) depends only on (seed, even-odd site index, component).

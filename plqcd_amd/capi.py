@@ -21,6 +21,17 @@ HOP_COMPAT, HOP_WILSON = 0, 1
 vp = C.c_void_p
 i4 = C.c_int * 4
 
+
+class CgParams(C.Structure):
+    _fields_ = [("kappa", C.c_double), ("tol", C.c_double), ("max_iter", C.c_int), ("delta", C.c_double),
+                ("use_graph", C.c_int), ("kappa_rhs", C.POINTER(C.c_double)), ("tol_rhs", C.POINTER(C.c_double))]
+
+
+class CgStats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("reliable_updates", C.c_int), ("converged", C.c_int),
+                ("true_residual", C.c_double), ("seconds", C.c_double), ("hop_applications", C.c_double)]
+
+
 # name -> (restype, argtypes)
 _PROTOS = {
     "init_plqcd": (C.c_int, [C.c_int, C.POINTER(C.c_char_p)]),
@@ -62,6 +73,10 @@ _PROTOS = {
     "plqcd_hip_field_download": (C.c_int, [vp, C.c_int, vp]),
     "plqcd_hip_field_bytes": (C.c_size_t, [vp]),
     "plqcd_hip_hop": (C.c_int, [vp, vp, vp, C.c_int, C.c_int]),
+    "plqcd_hip_apply_m": (C.c_int, [vp, vp, vp, C.POINTER(C.c_double), C.c_int]),
+    "plqcd_hip_cg_solve": (C.c_int, [vp, vp, vp, vp, C.POINTER(CgParams), C.POINTER(CgStats)]),
+    "plqcd_hip_hop_tuning": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "plqcd_hip_hop_get_tuning": (C.c_int, [C.c_int, C.POINTER(C.c_int)]),
     "plqcd_random_gauge_su3": (None, [vp, C.c_int, C.c_uint64]),
     "plqcd_random_spinor_gauss": (None, [vp, C.c_int, C.c_uint64]),
     "plqcd_unit_gauge": (None, [vp, C.c_int]),

@@ -1,0 +1,31 @@
+// Vector kernels of the solvers (device fields, per-RHS device scalars).
+// All are asynchronous on the field's lattice stream.  `partial`, when not
+// null, receives the chunk sums of the named norm (partial_doubles() doubles);
+// blas_norm_finish() turns them into one double per RHS.
+#pragma once
+
+#include "device_common.h"
+
+namespace plqcd {
+
+size_t partial_doubles(const plqcd_hip_field *f);
+int blas_norm_finish(const plqcd_hip_field *f, const double *partial, double *out);
+
+// x += alpha p ; r -= alpha Ap ; partial <- |r|^2   (all one precision)
+int blas_cg_update(plqcd_hip_field *x, const plqcd_hip_field *p, plqcd_hip_field *r, const plqcd_hip_field *ap,
+                   const double *alpha, double *partial);
+// p = r + beta p
+int blas_xpay(const plqcd_hip_field *r, const double *beta, plqcd_hip_field *p);
+// The masked kernels below touch only the RHS r with mask[r] != 0 (all when
+// mask is null); the chunk sums of untouched RHS are written as 0.
+// r = b - t ; partial <- |r|^2   (F64)
+int blas_residual(const plqcd_hip_field *b, const plqcd_hip_field *t, plqcd_hip_field *r, double *partial,
+                  const int *mask);
+// y += x ; x = 0 if zero_x   (any precisions)
+int blas_accum(plqcd_hip_field *y, plqcd_hip_field *x, int zero_x, const int *mask);
+// y = x ; partial <- |y|^2   (any precisions)
+int blas_copy(plqcd_hip_field *y, const plqcd_hip_field *x, double *partial, const int *mask);
+// y = 0 for RHS r with mask[r] != 0 (every RHS when mask is null)
+int blas_zero(plqcd_hip_field *y, const int *mask);
+
+}  // namespace plqcd

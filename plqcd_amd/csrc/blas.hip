@@ -1,0 +1,328 @@
+// Site-wise vector kernels for the solvers: every operand may be stored in
+// any precision; arithmetic is done in double.  Scalars are per right-hand
+// side and live in device memory, so a solver iteration needs no host round
+// trip.  Sums use the deterministic chunk reduction of reduce.h.
+#include "blas.h"
+#include "fields.h"
+#include "reduce.h"
+
+namespace {
+
+constexpr int kBlock = 256;
+
+// Lanes (js, r): tid = js * R + r; a block covers spb sites of one parity.
+struct BlasArgs {
+   int Vh, R, spb, nbp;  // nbp: blocks per parity
+   int nchunk;
+   double *partial;      // [R][2*nchunk] or null
+};
+
+__device__ __forceinline__ bool blas_site(const BlasArgs &a, int &p, int &j, int &js, int &r)
+{
+   const int blk = blockIdx.x;
+   p = blk / a.nbp;
+   r = threadIdx.x % a.R;
+   js = threadIdx.x / a.R;
+   j = (blk - p * a.nbp) * a.spb + js;
+   return j < a.Vh;
+}
+
+__device__ __forceinline__ void blas_reduce(const BlasArgs &a, double v, int p, int j, int js, int r)
+{
+   chunk_reduce<kBlock>(v, js, r, a.R, p * a.nchunk + j / REDUCE_CHUNK, j < a.Vh, a.partial + (size_t)r * 2 * a.nchunk);
+}
+
+template <int P>
+__device__ __forceinline__ void load_d(const SpinorIO<P> &f, int par, int j, d2 s[12])
+{
+   typename SpinorIO<P>::C t[12];
+   f.load(par, j, t);
+#pragma unroll
+   for (int c = 0; c < 12; c++) s[c] = d2{(double)t[c].x, (double)t[c].y};
+}
+
+template <int P>
+__device__ __forceinline__ void store_d(const SpinorIO<P> &f, int par, int j, const d2 s[12])
+{
+   typedef typename SpinorIO<P>::C C;
+   typedef typename SpinorIO<P>::real real;
+   C t[12];
+#pragma unroll
+   for (int c = 0; c < 12; c++) t[c] = C{(real)s[c].x, (real)s[c].y};
+   f.store(par, j, t);
+}
+
+__device__ __forceinline__ double norm2(const d2 s[12])
+{
+   double n = 0.0;
+#pragma unroll
+   for (int c = 0; c < 12; c++) n += s[c].x * s[c].x + s[c].y * s[c].y;
+   return n;
+}
+
+// x += alpha p ; r -= alpha Ap ; |r|^2
+template <int P>
+__global__ __launch_bounds__(kBlock) void k_cg_update(SpinorIO<P> x, SpinorIO<P> pv, SpinorIO<P> rv, SpinorIO<P> ap,
+                                                      const double *alpha, BlasArgs a)
+{
+   int p, j, js, r;
+   double n = 0.0;
+   if (blas_site(a, p, j, js, r)) {
+      const double al = alpha[r];
+      const SpinorIO<P> X = x.at_rhs(r), Pp = pv.at_rhs(r), Rr = rv.at_rhs(r), A = ap.at_rhs(r);
+      d2 xs[12], ps[12], rs[12], as[12];
+      load_d(X, p, j, xs);
+      load_d(Pp, p, j, ps);
+      load_d(Rr, p, j, rs);
+      load_d(A, p, j, as);
+#pragma unroll
+      for (int c = 0; c < 12; c++) {
+         xs[c] += al * ps[c];
+         rs[c] -= al * as[c];
+      }
+      store_d(X, p, j, xs);
+      store_d(Rr, p, j, rs);
+      n = norm2(rs);
+   }
+   if (a.partial) blas_reduce(a, n, p, j, js, r);
+}
+
+// p = r + beta p
+template <int P>
+__global__ __launch_bounds__(kBlock) void k_xpay(SpinorIO<P> rv, const double *beta, SpinorIO<P> pv, BlasArgs a)
+{
+   int p, j, js, r;
+   if (!blas_site(a, p, j, js, r)) return;
+   const double be = beta[r];
+   const SpinorIO<P> Rr = rv.at_rhs(r), Pp = pv.at_rhs(r);
+   d2 rs[12], ps[12];
+   load_d(Rr, p, j, rs);
+   load_d(Pp, p, j, ps);
+#pragma unroll
+   for (int c = 0; c < 12; c++) ps[c] = rs[c] + be * ps[c];
+   store_d(Pp, p, j, ps);
+}
+
+// r = b - t ; |r|^2
+template <int PB, int PT, int PR>
+__global__ __launch_bounds__(kBlock) void k_residual(SpinorIO<PB> bv, SpinorIO<PT> tv, SpinorIO<PR> rv, const int *mask,
+                                                     BlasArgs a)
+{
+   int p, j, js, r;
+   double n = 0.0;
+   if (blas_site(a, p, j, js, r) && (!mask || mask[r])) {
+      d2 bs[12], ts[12];
+      load_d(bv.at_rhs(r), p, j, bs);
+      load_d(tv.at_rhs(r), p, j, ts);
+#pragma unroll
+      for (int c = 0; c < 12; c++) bs[c] -= ts[c];
+      store_d(rv.at_rhs(r), p, j, bs);
+      n = norm2(bs);
+   }
+   if (a.partial) blas_reduce(a, n, p, j, js, r);
+}
+
+// y += x ; x = 0 (when zero_x) -- folds a low-precision partial solution into
+// the high-precision one
+template <int PY, int PX>
+__global__ __launch_bounds__(kBlock) void k_accum(SpinorIO<PY> yv, SpinorIO<PX> xv, int zero_x, const int *mask,
+                                                  BlasArgs a)
+{
+   int p, j, js, r;
+   if (!blas_site(a, p, j, js, r)) return;
+   if (mask && !mask[r]) return;
+   d2 ys[12], xs[12];
+   const SpinorIO<PY> Y = yv.at_rhs(r);
+   const SpinorIO<PX> X = xv.at_rhs(r);
+   load_d(Y, p, j, ys);
+   load_d(X, p, j, xs);
+#pragma unroll
+   for (int c = 0; c < 12; c++) ys[c] += xs[c];
+   store_d(Y, p, j, ys);
+   if (zero_x) {
+#pragma unroll
+      for (int c = 0; c < 12; c++) xs[c] = d2{0.0, 0.0};
+      store_d(X, p, j, xs);
+   }
+}
+
+// y = x (precision conversion) ; |y|^2 of the converted values
+template <int PY, int PX>
+__global__ __launch_bounds__(kBlock) void k_copy(SpinorIO<PY> yv, SpinorIO<PX> xv, const int *mask, BlasArgs a)
+{
+   int p, j, js, r;
+   double n = 0.0;
+   if (blas_site(a, p, j, js, r) && (!mask || mask[r])) {
+      d2 xs[12];
+      load_d(xv.at_rhs(r), p, j, xs);
+      store_d(yv.at_rhs(r), p, j, xs);
+      n = norm2(xs);
+   }
+   if (a.partial) blas_reduce(a, n, p, j, js, r);
+}
+
+// y = 0 for the right-hand sides with mask[r] != 0 (all when mask is null)
+template <int P>
+__global__ __launch_bounds__(kBlock) void k_zero(SpinorIO<P> yv, const int *mask, BlasArgs a)
+{
+   int p, j, js, r;
+   if (!blas_site(a, p, j, js, r)) return;
+   if (mask && !mask[r]) return;
+   d2 z[12];
+#pragma unroll
+   for (int c = 0; c < 12; c++) z[c] = d2{0.0, 0.0};
+   store_d(yv.at_rhs(r), p, j, z);
+}
+
+// out[r] = sum partial[r*stride + first + k], k < n, in a fixed order
+__global__ __launch_bounds__(256) void k_reduce(const double *partial, size_t stride, int first, int n, double *out)
+{
+   __shared__ double red[256];
+   const int r = blockIdx.x, t = threadIdx.x;
+   const double *p = partial + r * stride + first;
+   double s = 0.0;
+   for (int k = t; k < n; k += 256) s += p[k];
+   red[t] = s;
+   __syncthreads();
+   for (int w = 128; w > 0; w >>= 1) {
+      if (t < w) red[t] += red[t + w];
+      __syncthreads();
+   }
+   if (t == 0) out[r] = red[0];
+}
+
+BlasArgs make_args(const plqcd_hip_field *f, double *partial)
+{
+   BlasArgs a;
+   a.Vh = f->lat->Vh;
+   a.R = f->nrhs;
+   a.spb = REDUCE_CHUNK * (kBlock / (REDUCE_CHUNK * a.R));
+   a.nbp = (a.Vh + a.spb - 1) / a.spb;
+   a.nchunk = plqcd::reduce_chunks(a.Vh);
+   a.partial = partial;
+   return a;
+}
+
+int check_pair(const plqcd_hip_field *a, const plqcd_hip_field *b)
+{
+   if (!a || !b) return plqcd::set_error(1, "blas: null field");
+   if (a->lat != b->lat || a->nrhs != b->nrhs) return plqcd::set_error(1, "blas: fields of different shapes");
+   if (a->nrhs * REDUCE_CHUNK > kBlock) return plqcd::set_error(2, "blas: at most %d right-hand sides", kBlock / REDUCE_CHUNK);
+   return 0;
+}
+
+template <class F>
+int dispatch1(int prec, F f)
+{
+   switch (prec) {
+      case PLQCD_PREC_F64: f(std::integral_constant<int, PLQCD_PREC_F64>{}); break;
+      case PLQCD_PREC_F32: f(std::integral_constant<int, PLQCD_PREC_F32>{}); break;
+      default: f(std::integral_constant<int, PLQCD_PREC_F16>{});
+   }
+   PLQCD_HIP_TRY(hipGetLastError());
+   return 0;
+}
+
+template <class F>
+int dispatch2(int p1, int p2, F f)
+{
+   return dispatch1(p1, [&](auto a) { dispatch1(p2, [&](auto b) { f(a, b); }); });
+}
+
+dim3 grid_of(const BlasArgs &a) { return dim3(2 * a.nbp); }
+dim3 block_of(const BlasArgs &a) { return dim3(a.R * a.spb); }
+
+}  // namespace
+
+namespace plqcd {
+
+int reduce_partials(const double *partial, size_t stride, int first, int n, int R, double *out, hipStream_t st)
+{
+   hipLaunchKernelGGL(k_reduce, dim3(R), dim3(256), 0, st, partial, stride, first, n, out);
+   PLQCD_HIP_TRY(hipGetLastError());
+   return 0;
+}
+
+size_t partial_doubles(const plqcd_hip_field *f) { return (size_t)f->nrhs * 2 * reduce_chunks(f->lat->Vh); }
+
+int blas_norm_finish(const plqcd_hip_field *f, const double *partial, double *out)
+{
+   const int nc = reduce_chunks(f->lat->Vh);
+   return reduce_partials(partial, (size_t)2 * nc, 0, 2 * nc, f->nrhs, out, f->lat->stream);
+}
+
+int blas_cg_update(plqcd_hip_field *x, const plqcd_hip_field *p, plqcd_hip_field *r, const plqcd_hip_field *ap,
+                   const double *alpha, double *partial)
+{
+   if (int rc = check_pair(x, p) | check_pair(x, r) | check_pair(x, ap)) return rc;
+   if (x->prec != p->prec || x->prec != r->prec || x->prec != ap->prec)
+      return set_error(1, "blas_cg_update: precision mismatch");
+   const BlasArgs a = make_args(x, partial);
+   hipStream_t st = x->lat->stream;
+   return dispatch1(x->prec, [&](auto P) {
+      constexpr int PP = decltype(P)::value;
+      hipLaunchKernelGGL(k_cg_update<PP>, grid_of(a), block_of(a), 0, st, spinor_io<PP>(x), spinor_io<PP>(p),
+                         spinor_io<PP>(r), spinor_io<PP>(ap), alpha, a);
+   });
+}
+
+int blas_xpay(const plqcd_hip_field *r, const double *beta, plqcd_hip_field *p)
+{
+   if (int rc = check_pair(r, p)) return rc;
+   if (r->prec != p->prec) return set_error(1, "blas_xpay: precision mismatch");
+   const BlasArgs a = make_args(r, nullptr);
+   hipStream_t st = r->lat->stream;
+   return dispatch1(r->prec, [&](auto P) {
+      constexpr int PP = decltype(P)::value;
+      hipLaunchKernelGGL(k_xpay<PP>, grid_of(a), block_of(a), 0, st, spinor_io<PP>(r), beta, spinor_io<PP>(p), a);
+   });
+}
+
+int blas_residual(const plqcd_hip_field *b, const plqcd_hip_field *t, plqcd_hip_field *r, double *partial,
+                  const int *mask)
+{
+   if (int rc = check_pair(b, t) | check_pair(b, r)) return rc;
+   if (b->prec != PLQCD_PREC_F64 || t->prec != PLQCD_PREC_F64 || r->prec != PLQCD_PREC_F64)
+      return set_error(1, "blas_residual: F64 fields expected");
+   const BlasArgs a = make_args(b, partial);
+   hipLaunchKernelGGL((k_residual<PLQCD_PREC_F64, PLQCD_PREC_F64, PLQCD_PREC_F64>), grid_of(a), block_of(a), 0,
+                      b->lat->stream, spinor_io<PLQCD_PREC_F64>(b), spinor_io<PLQCD_PREC_F64>(t),
+                      spinor_io<PLQCD_PREC_F64>(r), mask, a);
+   PLQCD_HIP_TRY(hipGetLastError());
+   return 0;
+}
+
+int blas_accum(plqcd_hip_field *y, plqcd_hip_field *x, int zero_x, const int *mask)
+{
+   if (int rc = check_pair(y, x)) return rc;
+   const BlasArgs a = make_args(y, nullptr);
+   hipStream_t st = y->lat->stream;
+   return dispatch2(y->prec, x->prec, [&](auto PY, auto PX) {
+      constexpr int A = decltype(PY)::value, B = decltype(PX)::value;
+      hipLaunchKernelGGL((k_accum<A, B>), grid_of(a), block_of(a), 0, st, spinor_io<A>(y), spinor_io<B>(x), zero_x, mask, a);
+   });
+}
+
+int blas_copy(plqcd_hip_field *y, const plqcd_hip_field *x, double *partial, const int *mask)
+{
+   if (int rc = check_pair(y, x)) return rc;
+   const BlasArgs a = make_args(y, partial);
+   hipStream_t st = y->lat->stream;
+   return dispatch2(y->prec, x->prec, [&](auto PY, auto PX) {
+      constexpr int A = decltype(PY)::value, B = decltype(PX)::value;
+      hipLaunchKernelGGL((k_copy<A, B>), grid_of(a), block_of(a), 0, st, spinor_io<A>(y), spinor_io<B>(x), mask, a);
+   });
+}
+
+int blas_zero(plqcd_hip_field *y, const int *mask)
+{
+   if (int rc = check_pair(y, y)) return rc;
+   const BlasArgs a = make_args(y, nullptr);
+   hipStream_t st = y->lat->stream;
+   return dispatch1(y->prec, [&](auto P) {
+      constexpr int PP = decltype(P)::value;
+      hipLaunchKernelGGL(k_zero<PP>, grid_of(a), block_of(a), 0, st, spinor_io<PP>(y), mask, a);
+   });
+}
+
+}  // namespace plqcd

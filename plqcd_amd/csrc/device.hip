@@ -17,10 +17,11 @@ constexpr int kPackBlock = 256;
 
 // aos: n sites starting at even-odd index i0, 12 complex doubles each.
 template <int PREC>
-__global__ void pack_spinor(const d2 *__restrict__ aos, SpinorIO<PREC> f, int i0, int n)
+__global__ void pack_spinor(const d2 *__restrict__ aos, SpinorIO<PREC> f, int rhs, int i0, int n)
 {
    const int t = blockIdx.x * blockDim.x + threadIdx.x;
    if (t >= n) return;
+   f = f.at_rhs(rhs);
    const int i = i0 + t;
    const int p = i >= f.Vh;
    const int j = i - p * f.Vh;
@@ -35,10 +36,11 @@ __global__ void pack_spinor(const d2 *__restrict__ aos, SpinorIO<PREC> f, int i0
 }
 
 template <int PREC>
-__global__ void unpack_spinor(d2 *__restrict__ aos, SpinorIO<PREC> f, int i0, int n)
+__global__ void unpack_spinor(d2 *__restrict__ aos, SpinorIO<PREC> f, int rhs, int i0, int n)
 {
    const int t = blockIdx.x * blockDim.x + threadIdx.x;
    if (t >= n) return;
+   f = f.at_rhs(rhs);
    const int i = i0 + t;
    const int p = i >= f.Vh;
    const int j = i - p * f.Vh;
@@ -86,15 +88,6 @@ __global__ void pack_gauge(const d2 *__restrict__ aos, void *dst, int prec, int 
       *a = wa;
       *b = wb;
    }
-}
-
-template <int PREC>
-SpinorIO<PREC> spinor_io(const plqcd_hip_field *f, int rhs)
-{
-   SpinorIO<PREC> s;
-   s.base = reinterpret_cast<decltype(s.base)>(static_cast<char *>(f->data) + (size_t)rhs * f->rhs_bytes);
-   s.Vh = f->lat->Vh;
-   return s;
 }
 
 int ensure_stage(plqcd_hip_lattice *lat)
@@ -261,8 +254,7 @@ extern "C" int plqcd_hip_field_create(plqcd_hip_lattice *lat, int prec, int nrhs
    f->lat = lat;
    f->prec = prec;
    f->nrhs = nrhs;
-   f->rhs_bytes = (2 * plqcd::spinor_parity_bytes(prec, lat->Vh) + 255) & ~size_t(255);
-   f->bytes = f->rhs_bytes * nrhs;
+   f->bytes = (2 * plqcd::spinor_parity_bytes(prec, lat->Vh) * nrhs + 255) & ~size_t(255);
    hipError_t e = hipMalloc(&f->data, f->bytes);
    if (e != hipSuccess) {
       delete f;
@@ -299,15 +291,15 @@ int plqcd::field_upload_range(plqcd_hip_field *f, int rhs, const spinor *host, i
       switch (f->prec) {
          case PLQCD_PREC_F64:
             hipLaunchKernelGGL(pack_spinor<PLQCD_PREC_F64>, grid, block, 0, lat->stream, src,
-                               spinor_io<PLQCD_PREC_F64>(f, rhs), i0, n);
+                               spinor_io<PLQCD_PREC_F64>(f), rhs, i0, n);
             break;
          case PLQCD_PREC_F32:
             hipLaunchKernelGGL(pack_spinor<PLQCD_PREC_F32>, grid, block, 0, lat->stream, src,
-                               spinor_io<PLQCD_PREC_F32>(f, rhs), i0, n);
+                               spinor_io<PLQCD_PREC_F32>(f), rhs, i0, n);
             break;
          default:
             hipLaunchKernelGGL(pack_spinor<PLQCD_PREC_F16>, grid, block, 0, lat->stream, src,
-                               spinor_io<PLQCD_PREC_F16>(f, rhs), i0, n);
+                               spinor_io<PLQCD_PREC_F16>(f), rhs, i0, n);
       }
       PLQCD_HIP_TRY(hipGetLastError());
    }
@@ -330,15 +322,15 @@ int plqcd::field_download_range(const plqcd_hip_field *f, int rhs, spinor *host,
       switch (f->prec) {
          case PLQCD_PREC_F64:
             hipLaunchKernelGGL(unpack_spinor<PLQCD_PREC_F64>, grid, block, 0, lat->stream, dst,
-                               spinor_io<PLQCD_PREC_F64>(f, rhs), i0, n);
+                               spinor_io<PLQCD_PREC_F64>(f), rhs, i0, n);
             break;
          case PLQCD_PREC_F32:
             hipLaunchKernelGGL(unpack_spinor<PLQCD_PREC_F32>, grid, block, 0, lat->stream, dst,
-                               spinor_io<PLQCD_PREC_F32>(f, rhs), i0, n);
+                               spinor_io<PLQCD_PREC_F32>(f), rhs, i0, n);
             break;
          default:
             hipLaunchKernelGGL(unpack_spinor<PLQCD_PREC_F16>, grid, block, 0, lat->stream, dst,
-                               spinor_io<PLQCD_PREC_F16>(f, rhs), i0, n);
+                               spinor_io<PLQCD_PREC_F16>(f), rhs, i0, n);
       }
       PLQCD_HIP_TRY(hipGetLastError());
       PLQCD_HIP_TRY(
@@ -366,9 +358,10 @@ int plqcd::field_copy_range(const plqcd_hip_field *src, plqcd_hip_field *dst, in
 {
    if (src->prec != dst->prec || src->lat != dst->lat) return plqcd::set_error(1, "field copy: mismatch");
    const size_t pb = plqcd::spinor_parity_bytes(src->prec, src->lat->Vh);
-   if (src->prec == PLQCD_PREC_F16) return plqcd::set_error(1, "field copy: F16 parity copy not supported");
-   const char *s = static_cast<const char *>(src->data) + rhs * src->rhs_bytes + parity * pb;
-   char *d = static_cast<char *>(dst->data) + rhs * dst->rhs_bytes + parity * pb;
+   if (src->prec == PLQCD_PREC_F16 || src->nrhs != 1 || dst->nrhs != 1 || rhs != 0)
+      return plqcd::set_error(1, "field copy: only single-RHS F64/F32 parity copies are supported");
+   const char *s = static_cast<const char *>(src->data) + parity * pb;
+   char *d = static_cast<char *>(dst->data) + parity * pb;
    PLQCD_HIP_TRY(hipMemcpyAsync(d, s, pb, hipMemcpyDeviceToDevice, src->lat->stream));
    return 0;
 }

@@ -6,10 +6,10 @@
 // HBM layouts (Vh = V/2 sites per parity; j = index of a site inside its
 // parity, i.e. even-odd index minus parity*Vh, which keeps the reference's
 // site order):
-//   spinor F64  [rhs][parity][12 colour-spin][Vh]  double2   (192 B/site)
-//   spinor F32  [rhs][parity][6 pairs][Vh]        float4    ( 96 B/site)
-//   spinor F16  [rhs][parity][3 quads][Vh]        8 x fp16  ( 48 B/site)
-//               + [rhs][parity][Vh] float max-abs scale     (  4 B/site)
+//   spinor F64  [parity][12 colour-spin][Vh][rhs]  double2   (192 B/site)
+//   spinor F32  [parity][6 pairs][Vh][rhs]        float4    ( 96 B/site)
+//   spinor F16  [parity][3 quads][Vh][rhs]        8 x fp16  ( 48 B/site)
+//               + [parity][Vh][rhs] float max-abs scale     (  4 B/site)
 //   gauge  F64  [parity][mu][9 or 6][Vh]          double2
 //   gauge  F32  [parity][mu][9][Vh] float2  or  [parity][mu][3][Vh] float4
 //   gauge  F16  [parity][mu][Vh] 8 x fp16 (row 0 + row1.c0) and
@@ -55,7 +55,6 @@ struct plqcd_hip_field {
    int prec;
    int nrhs;
    void *data;
-   size_t rhs_bytes;  // bytes per right-hand side (both parities)
    size_t bytes;
 };
 

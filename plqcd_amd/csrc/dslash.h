@@ -1,11 +1,8 @@
-// Launch parameters of the hopping kernels.
+// Launch parameters of the hopping kernels and the internal entry point used
+// by the solvers.
 #pragma once
 
 #include "device_common.h"
-
-#ifndef HOP_BLOCK
-#define HOP_BLOCK 256
-#endif
 
 struct HopArgs {
    int L[4];     // extents x, y, z, t
@@ -13,7 +10,39 @@ struct HopArgs {
    int Vh;       // sites per parity
    int nparity;  // 1 (eo or oe) or 2 (full H)
    int parity0;  // output parity when nparity == 1
-   size_t rhs_stride;  // elements of the storage type between RHS
+   int order;    // block -> tile mapping (see logical_block)
+   int group;    // blocks per XCD run for order 2
+   int S;        // sites per parity in one t-slice (t-march)
+   int T;        // t-slices per thread (t-march)
+   int R;        // right-hand sides: lanes per site
+   int spb;      // sites per block
+   int nchunk;   // reduction chunks per parity
+   const double *kappa;  // per-RHS kappa of M = 1 - kappa H (EPI = 1)
+   double *partial;      // [R][2*nchunk] chunk sums of |out|^2 (NORM)
 };
 
+// Launch shape of the hopping kernels, process-wide.
+struct HopTuning {
+   int order, group, block;
+   int tmarch;  // t-slices per thread of the t-march kernel (0: gather kernel)
+};
+extern HopTuning g_tuning[3];  // per storage precision
+
 HopArgs make_hop_args(const plqcd_hip_lattice *lat, int parity);
+
+namespace plqcd {
+
+// What a launch computes besides H in:
+//   kappa   != null: out = in - kappa[r] * (H in)  (device array, one per RHS)
+//   dag     != 0:    H is replaced by H^+ = gamma5 H gamma5
+//   partial != null: chunk sums of |out|^2 into partial[r][2*nchunk]
+struct DslashOp {
+   int dag = 0;
+   const double *kappa = nullptr;
+   double *partial = nullptr;
+};
+
+int dslash(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip_field *out, int parity, int mode,
+           const DslashOp &op);
+
+}  // namespace plqcd

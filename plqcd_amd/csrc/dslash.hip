@@ -26,6 +26,7 @@
 // site with the same index inside its parity (dirac.c:643-644).
 #include "fields.h"
 #include "dslash.h"
+#include "reduce.h"
 
 namespace {
 
@@ -149,121 +150,386 @@ __device__ __forceinline__ void mul_udag(const C u[9], const C v[6], C r[6])
       }
 }
 
-template <int MU, int MODE, class SIO, class GIO>
+// (1 + SIGN gamma_mu) projection and reconstruction, SIGN = -1 or +1
+template <int MU, int SIGN, class C>
+__device__ __forceinline__ void proj(const C f[12], C h[6])
+{
+   if (SIGN < 0) proj_minus<MU>(f, h);
+   else proj_plus<MU>(f, h);
+}
+template <int MU, int SIGN, bool LOWER, class C>
+__device__ __forceinline__ void recon(C acc[12], const C a[6])
+{
+   if (SIGN < 0) recon_minus<MU, LOWER>(acc, a);
+   else recon_plus<MU>(acc, a);
+}
+
+// Forward term of direction MU for output site (p, j); jnb = site of x+mu.
+// H uses (1 - gamma_mu) forward and (1 + gamma_mu) backward; the adjoint
+// H^+ = gamma5 H gamma5 has the signs swapped (gamma5 anticommutes with
+// gamma_mu), so DAG only flips the projector signs.
+template <int MU, int MODE, bool DAG, class SIO, class GIO>
+__device__ __forceinline__ void hop_fwd(const SIO &in, const GIO &g, int p, int j, int jnb, typename SIO::C acc[12])
+{
+   typedef typename SIO::C C;
+   constexpr int SG = DAG ? 1 : -1;
+   C f[12], h[6], u[9], r[6];
+   in.load(1 - p, jnb, f);
+   proj<MU, SG>(f, h);
+   g.load(MODE == MODE_COMPAT ? 0 : p, MU, j, u);
+   mul_u(u, h, r);
+   recon<MU, SG, MODE != MODE_COMPAT>(acc, r);
+}
+
+// Backward term of direction MU; jnb = site of x-mu (its link U_mu(x-mu) too).
+template <int MU, bool DAG, class SIO, class GIO>
+__device__ __forceinline__ void hop_bwd(const SIO &in, const GIO &g, int p, int jnb, typename SIO::C acc[12])
+{
+   typedef typename SIO::C C;
+   constexpr int SG = DAG ? -1 : 1;
+   C f[12], k[6], u[9], r[6];
+   in.load(1 - p, jnb, f);
+   proj<MU, SG>(f, k);
+   g.load(1 - p, MU, jnb, u);
+   mul_udag(u, k, r);
+   recon<MU, SG, true>(acc, r);
+}
+
+template <int MU, int MODE, bool DAG, class SIO, class GIO>
 __device__ __forceinline__ void hop_dir(const SIO &in, const GIO &g, int p, int j, const Nbr &n,
                                         typename SIO::C acc[12])
 {
-   typedef typename SIO::C C;
-   const int q = 1 - p;
-   {  // forward: U_mu(x) (1 - gamma_mu) in(x + mu)
-      C f[12], h[6], u[9], r[6];
-      in.load(q, n.up[MU], f);
-      proj_minus<MU>(f, h);
-      g.load(MODE == MODE_COMPAT ? 0 : p, MU, j, u);
-      mul_u(u, h, r);
-      recon_minus<MU, MODE != MODE_COMPAT>(acc, r);
-   }
-   {  // backward: U_mu(x - mu)^+ (1 + gamma_mu) in(x - mu)
-      C f[12], k[6], u[9], r[6];
-      in.load(q, n.dn[MU], f);
-      proj_plus<MU>(f, k);
-      g.load(q, MU, n.dn[MU], u);
-      mul_udag(u, k, r);
-      recon_plus<MU>(acc, r);
-   }
+   hop_fwd<MU, MODE, DAG>(in, g, p, j, n.up[MU], acc);
+   hop_bwd<MU, DAG>(in, g, p, n.dn[MU], acc);
 }
 
-template <int MODE, class SIO, class GIO>
-__global__ __launch_bounds__(HOP_BLOCK) void hop_kernel(SIO in, SIO out, GIO g, HopArgs a)
+// Maps the hardware block index to a logical tile.  Blocks are handed to
+// the 8 XCDs round-robin (block b runs on XCD b % 8), and each XCD has its
+// own L2, so the order decides which L2 sees which part of the lattice:
+//   order 0: logical = b
+//   order 1: XCD k takes the k-th contiguous eighth of the lattice
+//   order 2: the lattice is cut into chunks of 8*group logical blocks; XCD k
+//            takes the k-th run of `group` blocks of every chunk, so all XCDs
+//            sweep the lattice together and neighbouring tiles share an L2
+// The output parity alternates between consecutive logical blocks, so a link
+// is read as a forward and as a backward link by nearby tiles.
+__device__ __forceinline__ int logical_block(const HopArgs &a)
+{
+   const int b = blockIdx.x, nb = gridDim.x;
+   if (a.order == 1) return (b & 7) * (nb >> 3) + (b >> 3);
+   if (a.order == 2) {
+      const int q = b >> 3, G = a.group;
+      return (q / G) * 8 * G + (b & 7) * G + (q % G);
+   }
+   return b;
+}
+
+// Epilogue: EPI = 0 stores H in; EPI = 1 stores in(x) - kappa_r * (H in)(x).
+// With NORM the site's |out|^2 joins the deterministic chunk reduction.
+template <int EPI, bool NORM, class SIO>
+__device__ __forceinline__ double epilogue(const SIO &in, const SIO &out, const HopArgs &a, int r, int p, int j,
+                                           typename SIO::C acc[12], const typename SIO::C *cen)
+{
+   typedef typename SIO::real real;
+   typedef typename SIO::C C;
+   if (EPI == 1) {
+      C c0[12];
+      if (cen == nullptr) in.load(p, j, c0);
+      const real kap = (real)a.kappa[r];
+#pragma unroll
+      for (int m = 0; m < 12; m++) acc[m] = (cen ? cen[m] : c0[m]) - kap * acc[m];
+   }
+   double nrm = 0.0;
+   if (NORM) {
+#pragma unroll
+      for (int m = 0; m < 12; m++) nrm += (double)acc[m].x * acc[m].x + (double)acc[m].y * acc[m].y;
+   }
+   out.store(p, j, acc);
+   return nrm;
+}
+
+template <int MODE, bool DAG, int EPI, bool NORM, int BLOCK, class SIO, class GIO>
+__global__ __launch_bounds__(BLOCK) void hop_kernel(SIO in, SIO out, GIO g, HopArgs a)
 {
    typedef typename SIO::C C;
-   // Full-lattice launches interleave the two output parities block by block,
-   // so that a link is read as a forward link and as a backward link by sites
-   // processed close together in time (and mostly served from L2).
-   int p, blk = blockIdx.x;
+   int p, blk = logical_block(a);
    if (a.nparity == 2) {
       p = blk & 1;
       blk >>= 1;
    } else {
       p = a.parity0;
    }
-   const int j = blk * HOP_BLOCK + threadIdx.x;
-   if (j >= a.Vh) return;
-   const size_t roff = (size_t)blockIdx.y * a.rhs_stride;
-   in.base += roff;
-   out.base += roff;
-
-   const Nbr n = neighbours(a, p, j);
-   C acc[12];
+   const int r = threadIdx.x % a.R;
+   const int js = threadIdx.x / a.R;
+   const int j = blk * a.spb + js;
+   double nrm = 0.0;
+   if (j < a.Vh) {
+      const SIO si = in.at_rhs(r), so = out.at_rhs(r);
+      const Nbr n = neighbours(a, p, j);
+      C acc[12];
 #pragma unroll
-   for (int c = 0; c < 12; c++) acc[c] = C{0, 0};
-   hop_dir<0, MODE>(in, g, p, j, n, acc);
-   hop_dir<1, MODE>(in, g, p, j, n, acc);
-   hop_dir<2, MODE>(in, g, p, j, n, acc);
-   hop_dir<3, MODE>(in, g, p, j, n, acc);
-   out.store(p, j, acc);
+      for (int c = 0; c < 12; c++) acc[c] = C{0, 0};
+      hop_dir<0, MODE, DAG>(si, g, p, j, n, acc);
+      hop_dir<1, MODE, DAG>(si, g, p, j, n, acc);
+      hop_dir<2, MODE, DAG>(si, g, p, j, n, acc);
+      hop_dir<3, MODE, DAG>(si, g, p, j, n, acc);
+      nrm = epilogue<EPI, NORM>(si, so, a, r, p, j, acc, (const C *)nullptr);
+   }
+   if (NORM) chunk_reduce<BLOCK>(nrm, js, r, a.R, p * a.nchunk + (blk * a.spb + js) / REDUCE_CHUNK,
+                                 j - (js & (REDUCE_CHUNK - 1)) < a.Vh, a.partial + (size_t)r * 2 * a.nchunk);
+}
+
+// t-march variant (wilson mode, full lattice): a thread owns one (x,y,z)
+// column and walks T consecutive t-slices.  Along t each spinor and each U_t
+// is loaded once: psi(t+1), read as the forward t-neighbour at step t, is kept
+// (it is the centre site of step t+1) and its projection, multiplied by
+// U_t(t+1)^+ at step t+1 (that step's forward link), gives the backward
+// t-term of step t+2.  The spatial directions read their neighbours from L2
+// as the gather kernel does.  The site parity alternates from step to step;
+// the index of the site inside its parity advances by S = L0/2*L1*L2 per
+// t-slice.  Terms are accumulated in the same order as in hop_kernel.
+template <bool DAG, int EPI, bool NORM, int BLOCK, class SIO, class GIO>
+__global__ __launch_bounds__(BLOCK) void hop_tmarch_kernel(SIO in, SIO out, GIO g, HopArgs a)
+{
+   typedef typename SIO::C C;
+   constexpr int SF = DAG ? 1 : -1;  // forward projector sign
+   const int S = a.S;
+   const int nbS = (S + a.spb - 1) / a.spb;
+   int lb = logical_block(a);
+   const int q = lb & 1;  // parity of the column's first site
+   lb >>= 1;
+   const int sb = lb % nbS;
+   const int chunk = lb / nbS;
+   const int r = threadIdx.x % a.R;
+   const int js = threadIdx.x / a.R;
+   const int c = sb * a.spb + js;
+   const bool valid = c < S;
+   const SIO si = in.at_rhs(r), so = out.at_rhs(r);
+
+   const int t0 = chunk * a.T;
+   const int cc = valid ? c : 0;
+   const int row = cc / a.Lxh;  // y + L1*z
+   const int xh = cc - row * a.Lxh;
+   const int y = row % a.L[1];
+   const int z = row / a.L[1];
+   const int x = 2 * xh + ((q + y + z + t0) & 1);
+   int cu[3], cd[3];
+   {
+      const int xp = (x + 1 == a.L[0]) ? 0 : x + 1;
+      const int xm = (x == 0) ? a.L[0] - 1 : x - 1;
+      cu[0] = row * a.Lxh + (xp >> 1);
+      cd[0] = row * a.Lxh + (xm >> 1);
+      const int ryp = (y == a.L[1] - 1) ? row - (a.L[1] - 1) : row + 1;
+      const int rym = (y == 0) ? row + (a.L[1] - 1) : row - 1;
+      const int rzp = (z == a.L[2] - 1) ? row - (a.L[2] - 1) * a.L[1] : row + a.L[1];
+      const int rzm = (z == 0) ? row + (a.L[2] - 1) * a.L[1] : row - a.L[1];
+      cu[1] = ryp * a.Lxh + xh;
+      cd[1] = rym * a.Lxh + xh;
+      cu[2] = rzp * a.Lxh + xh;
+      cd[2] = rzm * a.Lxh + xh;
+   }
+   const int Lt = a.L[3];
+   // prologue: cen = psi(t0); b = U_t(t0-1)^+ (1 -+ g_t) psi(t0-1)
+   C cen[12], b[6];
+   if (valid) {
+      C f[12], u[9], kk[6];
+      si.load(q, c + t0 * S, cen);
+      const int tm = (t0 == 0 ? Lt : t0) - 1;
+      si.load(q ^ 1, c + tm * S, f);
+      proj<3, -SF>(f, kk);
+      g.load(q ^ 1, 3, c + tm * S, u);
+      mul_udag(u, kk, b);
+   }
+   for (int s = 0; s < a.T; s++) {
+      const int t = t0 + s;
+      const int p = q ^ (s & 1);
+      const int base = t * S;
+      const int j = cc + base;
+      double nrm = 0.0;
+      if (valid) {
+         C acc[12];
+#pragma unroll
+         for (int m = 0; m < 12; m++) acc[m] = C{0, 0};
+         hop_fwd<0, PLQCD_HOP_WILSON, DAG>(si, g, p, j, cu[0] + base, acc);
+         hop_bwd<0, DAG>(si, g, p, cd[0] + base, acc);
+         hop_fwd<1, PLQCD_HOP_WILSON, DAG>(si, g, p, j, cu[1] + base, acc);
+         hop_bwd<1, DAG>(si, g, p, cd[1] + base, acc);
+         hop_fwd<2, PLQCD_HOP_WILSON, DAG>(si, g, p, j, cu[2] + base, acc);
+         hop_bwd<2, DAG>(si, g, p, cd[2] + base, acc);
+         C f[12];
+         {
+            C h[6], u[9], rr[6], kk[6];
+            const int tp = (t + 1 == Lt) ? 0 : t + 1;
+            si.load(p ^ 1, c + tp * S, f);
+            proj<3, SF>(f, h);
+            g.load(p, 3, j, u);
+            mul_u(u, h, rr);
+            recon<3, SF, true>(acc, rr);
+            recon<3, -SF, true>(acc, b);
+            proj<3, -SF>(cen, kk);
+            mul_udag(u, kk, b);
+         }
+         nrm = epilogue<EPI, NORM>(si, so, a, r, p, j, acc, cen);
+#pragma unroll
+         for (int m = 0; m < 12; m++) cen[m] = f[m];
+      }
+      if (NORM) chunk_reduce<BLOCK>(nrm, js, r, a.R, p * a.nchunk + (sb * a.spb + js + base) / REDUCE_CHUNK, valid,
+                                    a.partial + (size_t)r * 2 * a.nchunk);
+   }
+}
+
+template <int MODE, bool DAG, int EPI, bool NORM, int BLOCK, class S, class G>
+void launch_one(bool march, dim3 grid, dim3 block, hipStream_t st, S si, S so, G gg, const HopArgs &a)
+{
+   if (march) {
+      if constexpr (MODE == PLQCD_HOP_WILSON)
+         hipLaunchKernelGGL((hop_tmarch_kernel<DAG, EPI, NORM, BLOCK, S, G>), grid, block, 0, st, si, so, gg, a);
+   } else {
+      hipLaunchKernelGGL((hop_kernel<MODE, DAG, EPI, NORM, BLOCK, S, G>), grid, block, 0, st, si, so, gg, a);
+   }
+}
+
+template <int MODE, bool DAG, int EPI, bool NORM, class S, class G>
+void launch_block(int maxblock, bool march, dim3 grid, dim3 block, hipStream_t st, S si, S so, G gg,
+                  const HopArgs &a)
+{
+   if (maxblock <= 128) launch_one<MODE, DAG, EPI, NORM, 128>(march, grid, block, st, si, so, gg, a);
+   else if (maxblock <= 256) launch_one<MODE, DAG, EPI, NORM, 256>(march, grid, block, st, si, so, gg, a);
+   else launch_one<MODE, DAG, EPI, NORM, 512>(march, grid, block, st, si, so, gg, a);
 }
 
 template <int PREC, int LINKS>
-int launch_t(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip_field *out, int parity, int mode)
+int launch_t(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip_field *out, int parity, int mode,
+             const plqcd::DslashOp &op)
 {
    const plqcd_hip_lattice *lat = g->lat;
    HopArgs a = make_hop_args(lat, parity);
    typedef SpinorIO<PREC> S;
    typedef GaugeIO<PREC, LINKS> G;
-   S si, so;
-   si.base = reinterpret_cast<decltype(si.base)>(in->data);
-   so.base = reinterpret_cast<decltype(so.base)>(out->data);
-   si.Vh = so.Vh = lat->Vh;
+   const S si = spinor_io<PREC>(in), so = spinor_io<PREC>(out);
    G gg;
    gg.base = reinterpret_cast<decltype(gg.base)>(g->data);
    gg.Vh = lat->Vh;
-   a.rhs_stride = in->rhs_bytes / sizeof(*si.base);
-   const int nblk = (lat->Vh + HOP_BLOCK - 1) / HOP_BLOCK;
-   dim3 grid(nblk * a.nparity, in->nrhs);
-   if (mode == PLQCD_HOP_COMPAT)
-      hipLaunchKernelGGL((hop_kernel<PLQCD_HOP_COMPAT, S, G>), grid, dim3(HOP_BLOCK), 0, lat->stream, si, so, gg, a);
-   else
-      hipLaunchKernelGGL((hop_kernel<PLQCD_HOP_WILSON, S, G>), grid, dim3(HOP_BLOCK), 0, lat->stream, si, so, gg, a);
+   const int R = in->nrhs;
+   const HopTuning &tu = g_tuning[PREC];
+   // block shape: R lanes per site and a whole number of reduction chunks
+   int target = tu.block;
+   if (R * REDUCE_CHUNK > target) {
+      target = 128;
+      while (target < R * REDUCE_CHUNK) target *= 2;
+   }
+   if (target > 512)
+      return plqcd::set_error(2, "hop: %d right-hand sides exceed the per-launch limit of %d", R, 512 / REDUCE_CHUNK);
+   a.R = R;
+   a.spb = REDUCE_CHUNK * (target / (REDUCE_CHUNK * R));
+   const int threads = R * a.spb;
+   const int maxblock = threads <= 128 ? 128 : threads <= 256 ? 256 : 512;
+   a.kappa = op.kappa;
+   a.partial = op.partial;
+   a.nchunk = plqcd::reduce_chunks(lat->Vh);
+   const bool dag = op.dag != 0;
+   const int epi = op.kappa ? 1 : 0;
+   const bool norm = op.partial != nullptr;
+   if ((dag || epi || norm) && mode != PLQCD_HOP_WILSON)
+      return plqcd::set_error(2, "hop: M, M^+ and norms need the wilson mode");
+   if (norm && a.nparity != 2) return plqcd::set_error(2, "hop: norms are only fused into full-lattice launches");
+   const int T = tu.tmarch;
+   // the t-march kernel serves full-lattice wilson applications whose time
+   // extent is a multiple of T (and, with a norm, whose t-slices hold whole
+   // reduction chunks)
+   const bool march = T > 0 && mode == PLQCD_HOP_WILSON && a.nparity == 2 && lat->L[3] % T == 0 &&
+                      (!norm || a.S % REDUCE_CHUNK == 0);
+   a.T = march ? T : 1;
+   const int nblk = march ? ((a.S + a.spb - 1) / a.spb) * (lat->L[3] / T) : (lat->Vh + a.spb - 1) / a.spb;
+   const int nb = nblk * a.nparity;
+   // the remapped orders need whole rounds of 8 XCDs (and of 8*group blocks)
+   a.order = tu.order;
+   a.group = tu.group > 0 ? tu.group : 1;
+   if ((a.order == 1 && nb % 8) || (a.order == 2 && nb % (8 * a.group))) a.order = 0;
+   const dim3 grid(nb), block(threads);
+   hipStream_t st = lat->stream;
+   if (mode == PLQCD_HOP_COMPAT) {
+      launch_block<PLQCD_HOP_COMPAT, false, 0, false>(maxblock, false, grid, block, st, si, so, gg, a);
+   } else if (!dag) {
+      if (epi == 0) launch_block<PLQCD_HOP_WILSON, false, 0, false>(maxblock, march, grid, block, st, si, so, gg, a);
+      else if (!norm) launch_block<PLQCD_HOP_WILSON, false, 1, false>(maxblock, march, grid, block, st, si, so, gg, a);
+      else launch_block<PLQCD_HOP_WILSON, false, 1, true>(maxblock, march, grid, block, st, si, so, gg, a);
+   } else {
+      if (epi == 0) launch_block<PLQCD_HOP_WILSON, true, 0, false>(maxblock, march, grid, block, st, si, so, gg, a);
+      else if (!norm) launch_block<PLQCD_HOP_WILSON, true, 1, false>(maxblock, march, grid, block, st, si, so, gg, a);
+      else launch_block<PLQCD_HOP_WILSON, true, 1, true>(maxblock, march, grid, block, st, si, so, gg, a);
+   }
    PLQCD_HIP_TRY(hipGetLastError());
    return 0;
 }
 
 }  // namespace
 
+// Defaults per storage precision (F64, F32, F16), from sweeps at 32^3 x 64.
+HopTuning g_tuning[3] = {{1, 1, 256, 16}, {1, 1, 256, 16}, {1, 1, 64, 4}};
+
 HopArgs make_hop_args(const plqcd_hip_lattice *lat, int parity)
 {
-   HopArgs a;
+   HopArgs a{};
    for (int mu = 0; mu < 4; mu++) a.L[mu] = lat->L[mu];
    a.Lxh = lat->Lxh;
    a.Vh = lat->Vh;
    a.nparity = (parity == PLQCD_PARITY_ALL) ? 2 : 1;
    a.parity0 = (parity == PLQCD_PARITY_ODD) ? 1 : 0;
-   a.rhs_stride = 0;
+   a.order = 0;
+   a.group = 1;
+   a.S = lat->Vh / lat->L[3];
+   a.T = 1;
+   a.R = 1;
+   a.spb = 256;
    return a;
+}
+
+int plqcd::dslash(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip_field *out, int parity, int mode,
+                  const DslashOp &op)
+{
+   if (!g || !in || !out) return plqcd::set_error(1, "hop: null argument");
+   if (in == out) return plqcd::set_error(1, "hop: in and out must be distinct fields");
+   if (in->lat != g->lat || out->lat != g->lat)
+      return plqcd::set_error(1, "hop: fields and gauge belong to different lattices");
+   if (in->prec != g->prec || out->prec != g->prec)
+      return plqcd::set_error(1, "hop: precision mismatch (gauge %d, in %d, out %d)", g->prec, in->prec, out->prec);
+   if (in->nrhs != out->nrhs) return plqcd::set_error(1, "hop: RHS count mismatch");
+   if (parity < PLQCD_PARITY_EVEN || parity > PLQCD_PARITY_ALL) return plqcd::set_error(1, "hop: bad parity %d", parity);
+   if (mode != PLQCD_HOP_COMPAT && mode != PLQCD_HOP_WILSON) return plqcd::set_error(1, "hop: bad mode %d", mode);
+   switch (g->prec * 100 + g->links) {
+      case PLQCD_PREC_F64 * 100 + 18: return launch_t<PLQCD_PREC_F64, 18>(g, in, out, parity, mode, op);
+      case PLQCD_PREC_F64 * 100 + 12: return launch_t<PLQCD_PREC_F64, 12>(g, in, out, parity, mode, op);
+      case PLQCD_PREC_F32 * 100 + 18: return launch_t<PLQCD_PREC_F32, 18>(g, in, out, parity, mode, op);
+      case PLQCD_PREC_F32 * 100 + 12: return launch_t<PLQCD_PREC_F32, 12>(g, in, out, parity, mode, op);
+      case PLQCD_PREC_F16 * 100 + 12: return launch_t<PLQCD_PREC_F16, 12>(g, in, out, parity, mode, op);
+      default: return plqcd::set_error(1, "hop: unsupported gauge format");
+   }
 }
 
 extern "C" int plqcd_hip_hop(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip_field *out, int parity,
                              int mode)
 {
-   if (!g || !in || !out) return plqcd::set_error(1, "plqcd_hip_hop: null argument");
-   if (in == out) return plqcd::set_error(1, "plqcd_hip_hop: in and out must be distinct fields");
-   if (in->lat != g->lat || out->lat != g->lat)
-      return plqcd::set_error(1, "plqcd_hip_hop: fields and gauge belong to different lattices");
-   if (in->prec != g->prec || out->prec != g->prec)
-      return plqcd::set_error(1, "plqcd_hip_hop: precision mismatch (gauge %d, in %d, out %d)", g->prec, in->prec,
-                              out->prec);
-   if (in->nrhs != out->nrhs) return plqcd::set_error(1, "plqcd_hip_hop: RHS count mismatch");
-   if (parity < PLQCD_PARITY_EVEN || parity > PLQCD_PARITY_ALL)
-      return plqcd::set_error(1, "plqcd_hip_hop: bad parity %d", parity);
-   if (mode != PLQCD_HOP_COMPAT && mode != PLQCD_HOP_WILSON)
-      return plqcd::set_error(1, "plqcd_hip_hop: bad mode %d", mode);
-   switch (g->prec * 100 + g->links) {
-      case PLQCD_PREC_F64 * 100 + 18: return launch_t<PLQCD_PREC_F64, 18>(g, in, out, parity, mode);
-      case PLQCD_PREC_F64 * 100 + 12: return launch_t<PLQCD_PREC_F64, 12>(g, in, out, parity, mode);
-      case PLQCD_PREC_F32 * 100 + 18: return launch_t<PLQCD_PREC_F32, 18>(g, in, out, parity, mode);
-      case PLQCD_PREC_F32 * 100 + 12: return launch_t<PLQCD_PREC_F32, 12>(g, in, out, parity, mode);
-      case PLQCD_PREC_F16 * 100 + 12: return launch_t<PLQCD_PREC_F16, 12>(g, in, out, parity, mode);
-      default: return plqcd::set_error(1, "plqcd_hip_hop: unsupported gauge format");
-   }
+   return plqcd::dslash(g, in, out, parity, mode, plqcd::DslashOp{});
+}
+
+extern "C" int plqcd_hip_hop_tuning(int prec, int order, int group, int block, int tmarch)
+{
+   if (prec < -1 || prec > PLQCD_PREC_F16 || order < 0 || order > 2 || group < 1 || group > 4096 ||
+       (block != 64 && block != 128 && block != 256 && block != 512) || tmarch < 0 || tmarch > 4096)
+      return plqcd::set_error(1, "plqcd_hip_hop_tuning: bad parameters (prec %d, order %d, group %d, block %d, "
+                                 "tmarch %d)", prec, order, group, block, tmarch);
+   for (int p = 0; p < 3; p++)
+      if (prec < 0 || p == prec) g_tuning[p] = HopTuning{order, group, block, tmarch};
+   return 0;
+}
+
+extern "C" int plqcd_hip_hop_get_tuning(int prec, int out[4])
+{
+   if (prec < 0 || prec > PLQCD_PREC_F16 || !out) return plqcd::set_error(1, "plqcd_hip_hop_get_tuning: bad arguments");
+   out[0] = g_tuning[prec].order;
+   out[1] = g_tuning[prec].group;
+   out[2] = g_tuning[prec].block;
+   out[3] = g_tuning[prec].tmarch;
+   return 0;
 }

@@ -225,3 +225,31 @@ def test_benchmark_shape_32c64_f32_error(hp, orc):
         err = site_rel_err(device_hop(hp, L, q, u, hp.HOP_WILSON, prec=1, links=links), ref)
         print(f"32^3x64 FP32 {links}-real links: {err:.3e}")
         assert err < TOL32
+
+
+# ---------------------------------------------------------- launch shapes
+
+TUNINGS = [(0, 1, 256, 0), (1, 1, 128, 0), (2, 4, 64, 0), (2, 16, 512, 0),
+           (2, 16, 256, 2), (1, 1, 128, 4), (2, 8, 64, 8), (0, 1, 256, 16)]
+
+
+@pytest.mark.parametrize("tuning", TUNINGS)
+@pytest.mark.parametrize("prec,links,tol", [(0, 18, TOL64), (1, 12, TOL32), (2, 12, TOL16)])
+def test_launch_shapes_match_oracle(hp, orc, tuning, prec, links, tol):
+    """Every block order, block size and t-march length gives the operator."""
+    import ctypes as C
+    from plqcd_amd import capi
+    L = (8, 4, 6, 16)
+    q, u = inputs(hp, L)
+    ref = orc.hopping_full(q, u, L, orc.MODE_WILSON)
+    lib = capi.lib()
+    saved = (C.c_int * 4)()
+    lib.plqcd_hip_hop_get_tuning(prec, saved)
+    assert lib.plqcd_hip_hop_tuning(prec, *tuning) == 0
+    try:
+        for mode in (1, 0):
+            out = device_hop(hp, L, q, u, mode, prec=prec, links=links)
+            r = ref if mode == 1 else orc.hopping_full(q, u, L, orc.MODE_COMPAT)
+            assert site_rel_err(out, r) < tol, (tuning, mode)
+    finally:
+        lib.plqcd_hip_hop_tuning(prec, *saved)

@@ -1,0 +1,120 @@
+"""GPU solver tests through the C ABI.  The reference ships no solver, so
+results are pinned by the CPU oracle: the FP64 true residual
+||b - M^+M x|| / ||b|| is recomputed on the CPU with the oracle's hopping
+matrix and must meet the requested tolerance (x 1.05 for the different
+summation order), and M, M^+ are compared site by site (FP64, 1e-13)."""
+import numpy as np
+import pytest
+
+from conftest import site_rel_err
+
+pytestmark = pytest.mark.gpu
+KAPPA = 0.124
+
+
+def setup(hp, L, nrhs=1, gseed=41, sseed=42):
+    V = int(np.prod(L))
+    u = hp.random_gauge_su3(V, gseed)
+    bs = [hp.random_spinor(V, sseed + r) for r in range(nrhs)]
+    lat = hp.Lattice(L)
+    g = hp.Gauge(lat).upload(u)
+    b = hp.Field(lat, nrhs=nrhs)
+    for r, q in enumerate(bs):
+        b.upload(q, r)
+    return lat, g, b, u, bs
+
+
+@pytest.mark.parametrize("dagger", [False, True])
+def test_apply_m_matches_oracle(hp, dagger):
+    from oracle import solver as osv
+    from plqcd_amd import solver as sv
+    L = (4, 6, 4, 8)
+    lat, g, b, u, bs = setup(hp, L, nrhs=3)
+    out = hp.Field(lat, nrhs=3)
+    kaps = [0.12, 0.124, 0.1245]
+    sv.apply_m(g, b, out, kaps, dagger)
+    for r in range(3):
+        ref = osv.apply_m(bs[r], u, L, kaps[r], dagger)
+        assert site_rel_err(out.download(r), ref) < 1e-13
+
+
+@pytest.mark.parametrize("sloppy", [None, (1, 12), (1, 18), (2, 12)])
+def test_cg_true_residual(hp, sloppy):
+    from oracle import solver as osv
+    from plqcd_amd import solver as sv
+    L = (8, 8, 8, 8)
+    lat, g, b, u, bs = setup(hp, L)
+    x = hp.Field(lat)
+    gs = hp.Gauge(lat, *sloppy).upload(u) if sloppy else None
+    st = sv.cg_solve(g, b, x, KAPPA, 1e-10, sloppy=gs)[0]
+    print(sloppy, st)
+    assert st["converged"] and st["true_residual"] <= 1e-10
+    res = osv.true_residual(bs[0], x.download(), u, L, KAPPA)
+    assert res <= 1.05e-10, res
+    _, it_ref = osv.cg(bs[0], u, L, KAPPA, 1e-10)
+    if sloppy is None:
+        assert abs(st["iterations"] - it_ref) <= 2
+    else:
+        assert st["iterations"] <= 2 * it_ref
+
+
+def test_cg_batch_equals_alone_bitwise(hp):
+    """Each RHS follows the same iterates alone and in a batch."""
+    from plqcd_amd import solver as sv
+    L = (8, 4, 4, 8)
+    kaps, tols = [0.12, 0.124, 0.1245], [1e-8, 1e-10, 1e-12]
+    lat, g, b, u, bs = setup(hp, L, nrhs=3)
+    gs = hp.Gauge(lat, 1, 12).upload(u)
+    for sloppy in (None, gs):
+        x = hp.Field(lat, nrhs=3)
+        st = sv.cg_solve(g, b, x, kaps, tols, sloppy=sloppy)
+        for r in range(3):
+            b1 = hp.Field(lat).upload(bs[r])
+            x1 = hp.Field(lat)
+            st1 = sv.cg_solve(g, b1, x1, kaps[r], tols[r], sloppy=sloppy)[0]
+            assert st1["iterations"] == st[r]["iterations"]
+            assert st1["true_residual"] == st[r]["true_residual"]
+            assert np.array_equal(x1.download(), x.download(r))
+
+
+def test_cg_graph_replay_equals_eager(hp):
+    from plqcd_amd import solver as sv
+    L = (4, 4, 8, 8)
+    lat, g, b, u, bs = setup(hp, L, nrhs=2)
+    gs = hp.Gauge(lat, 1, 12).upload(u)
+    xa, xb = hp.Field(lat, nrhs=2), hp.Field(lat, nrhs=2)
+    sa = sv.cg_solve(g, b, xa, KAPPA, 1e-10, sloppy=gs, use_graph=True)
+    sb = sv.cg_solve(g, b, xb, KAPPA, 1e-10, sloppy=gs, use_graph=False)
+    for r in range(2):
+        assert sa[r]["iterations"] == sb[r]["iterations"]
+        assert np.array_equal(xa.download(r), xb.download(r))
+
+
+def test_cg_zero_source_and_errors(hp):
+    from plqcd_amd import capi, solver as sv
+    L = (4, 4, 4, 4)
+    lat, g, b, u, bs = setup(hp, L, nrhs=2)
+    b.upload(hp.spinor_field(256), 1)
+    x = hp.Field(lat, nrhs=2)
+    st = sv.cg_solve(g, b, x, KAPPA, 1e-10)
+    assert st[1]["converged"] and st[1]["iterations"] == 0
+    assert np.all(x.download(1) == 0)
+    assert st[0]["converged"]
+    with pytest.raises(capi.PlqcdError):
+        sv.cg_solve(g, b, hp.Field(lat, nrhs=1), KAPPA, 1e-10)
+    with pytest.raises(capi.PlqcdError):
+        sv.cg_solve(hp.Gauge(lat, 1, 18).upload(u), b, x, KAPPA, 1e-10)
+
+
+def test_cg_benchmark_shape_32c64(hp):
+    """32^3 x 64: mixed FP32 solve, residual checked by the CPU oracle."""
+    from oracle import solver as osv
+    from plqcd_amd import solver as sv
+    L = (32, 32, 32, 64)
+    lat, g, b, u, bs = setup(hp, L)
+    gs = hp.Gauge(lat, 1, 12).upload(u)
+    x = hp.Field(lat)
+    st = sv.cg_solve(g, b, x, KAPPA, 1e-10, sloppy=gs)[0]
+    print(st)
+    assert st["converged"]
+    assert osv.true_residual(bs[0], x.download(), u, L, KAPPA) <= 1.05e-10
