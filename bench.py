@@ -109,6 +109,246 @@ def cpu_baseline(L, seconds: float):
             "sample": f"C/OpenMP restatement eo+oe ({cores} threads), 32^3x64 FP64, {n} applications of H in {t:.1f}s"}
 
 
+def solve_compare(L, nrhs=12, sloppy=(2, 12), kappa=0.124, tol=1e-10):
+    """Plain FP64 CG on one RHS vs the mixed-precision CG on a batch of nrhs,
+    same gauge field and sources, true residual <= tol.  Returns a dict."""
+    from plqcd_amd import hopping as hp, solver as sv
+    V = int(np.prod(L))
+    u = hp.random_gauge_su3(V, 1)
+    lat = hp.Lattice(L)
+    g = hp.Gauge(lat).upload(u)
+    gs = hp.Gauge(lat, *sloppy).upload(u)
+    out = {}
+    for label, R, sl in (("fp64_1rhs", 1, None), (f"mixed_{PREC_NAMES[sloppy[0]]}_{nrhs}rhs", nrhs, gs)):
+        b = hp.Field(lat, 0, R)
+        x = hp.Field(lat, 0, R)
+        for r in range(R):
+            b.upload(hp.random_spinor(V, 100 + r), r)
+        sv.cg_solve(g, b, x, kappa, tol, sloppy=sl)  # graph capture and first-touch costs
+        lat.sync()
+        t0 = time.perf_counter()
+        st = sv.cg_solve(g, b, x, kappa, tol, sloppy=sl)
+        t = time.perf_counter() - t0
+        out[label] = {"rhs": R, "seconds": round(t, 4), "solves_per_s": round(R / t, 3),
+                      "iterations": max(s_["iterations"] for s_ in st),
+                      "reliable_updates": max(s_["reliable_updates"] for s_ in st),
+                      "max_true_residual": max(s_["true_residual"] for s_ in st),
+                      "all_converged": all(s_["converged"] for s_ in st)}
+        b.close(); x.close()
+    keys = list(out)
+    out["ratio_solves_per_s"] = round(out[keys[1]]["solves_per_s"] / out[keys[0]]["solves_per_s"], 3)
+    out["lattice"] = list(L)
+    out["kappa"], out["tol"] = kappa, tol
+    gs.close(); g.close(); lat.close()
+    return out
+
+
+def emit(rec):
+    print(json.dumps(rec), flush=True)
+
+
+def time_hop(hp, g, fi, fo, n=20):
+    lat = g.lat
+    for _ in range(3):
+        hp.hop(g, fi, fo, hp.PARITY_ALL, hp.HOP_WILSON)
+    lat.timer_start()
+    for _ in range(n):
+        hp.hop(g, fi, fo, hp.PARITY_ALL, hp.HOP_WILSON)
+    return lat.timer_stop() / n
+
+
+def solver_suite():
+    """configs[2] (48^3x96 CG: FP64 vs mixed), configs[3] (multi-RHS hop and
+    CG at 32^3x64, ragged persistent batch, autotuner).  One JSON line each."""
+    import ctypes as C
+    from plqcd_amd import capi, hopping as hp, solver as sv
+    lib = capi.lib()
+    # --- configs[2]: single-RHS CG at 48^3 x 96, kappa 0.124, 1e-10
+    L = (48, 48, 48, 96)
+    V = int(np.prod(L))
+    u = hp.random_gauge_su3(V, 1)
+    lat = hp.Lattice(L)
+    g = hp.Gauge(lat).upload(u)
+    b = hp.Field(lat).upload(hp.random_spinor(V, 2))
+    x = hp.Field(lat)
+    for label, sl in (("fp64", None), ("mixed_f32_l12", (1, 12)), ("mixed_f16_l12", (2, 12))):
+        gs = hp.Gauge(lat, *sl).upload(u) if sl else None
+        sv.cg_solve(g, b, x, 0.124, 1e-10, sloppy=gs)
+        t0 = time.perf_counter()
+        st = sv.cg_solve(g, b, x, 0.124, 1e-10, sloppy=gs)[0]
+        t = time.perf_counter() - t0
+        emit({"suite": "solver", "case": f"cg_48c96_{label}", "seconds": round(t, 4), "iterations": st["iterations"],
+              "reliable_updates": st["reliable_updates"], "true_residual": st["true_residual"],
+              "converged": st["converged"]})
+        if gs:
+            gs.close()
+    x.close(); b.close(); g.close(); lat.close()
+    del u
+    # --- configs[3]: multi-RHS at 32^3 x 64
+    L = (32, 32, 32, 64)
+    V = int(np.prod(L))
+    u = hp.random_gauge_su3(V, 1)
+    lat = hp.Lattice(L)
+    for prec, links in ((0, 18), (1, 12), (2, 12)):
+        g = hp.Gauge(lat, prec, links).upload(u)
+        for R in (1, 4, 12, 24):
+            fi, fo = hp.Field(lat, prec, R), hp.Field(lat, prec, R)
+            for r in range(R):
+                fi.upload(hp.random_spinor(V, 10 + r), r)
+            ms = time_hop(hp, g, fi, fo)
+            emit({"suite": "solver", "case": f"hop_32c64_{PREC_NAMES[prec]}_l{links}_R{R}",
+                  "us_per_application": round(ms * 1e3, 1), "us_per_rhs": round(ms * 1e3 / R, 1),
+                  "gflops": round(FLOP_PER_SITE * V * R / ms / 1e6, 1)})
+            fi.close(); fo.close()
+        g.close()
+    for R in (1, 4, 12, 24):
+        for sl in ((1, 12), (2, 12)):
+            rec = solve_compare(L, nrhs=R, sloppy=sl)
+            rec.update({"suite": "solver", "case": f"cg_32c64_mixed_{PREC_NAMES[sl[0]]}_R{R}_vs_fp64_R1"})
+            emit(rec)
+    # ragged batch: 16^3x32, 24^3x48, 32^3x64, each 12 RHS, one persistent launch
+    shapes = [(16, 16, 16, 32), (24, 24, 24, 48), (32, 32, 32, 64)]
+    lats, gs_, fis, fos = [], [], [], []
+    for k, Ls in enumerate(shapes):
+        Vs = int(np.prod(Ls))
+        la = hp.Lattice(Ls)
+        la.set_stream(lib.plqcd_hip_lattice_stream(lat.h))
+        lats.append(la)
+        gs_.append(hp.Gauge(la, 1, 12).upload(hp.random_gauge_su3(Vs, 5 + k)))
+        fi, fo = hp.Field(la, 1, 12), hp.Field(la, 1, 12)
+        for r in range(12):
+            fi.upload(hp.random_spinor(Vs, 50 + r), r)
+        fis.append(fi); fos.append(fo)
+    n = len(shapes)
+    garr = (C.c_void_p * n)(*[x_.h.value for x_ in gs_])
+    iarr = (C.c_void_p * n)(*[x_.h.value for x_ in fis])
+    oarr = (C.c_void_p * n)(*[x_.h.value for x_ in fos])
+    for _ in range(3):
+        capi.check(lib.plqcd_hip_hop_batch(n, garr, iarr, oarr, 1), "hop_batch")
+    lat.timer_start()
+    for _ in range(10):
+        capi.check(lib.plqcd_hip_hop_batch(n, garr, iarr, oarr, 1), "hop_batch")
+    t_batch = lat.timer_stop() / 10
+    lat.timer_start()
+    for _ in range(10):
+        for k in range(n):
+            hp.hop(gs_[k], fis[k], fos[k], hp.PARITY_ALL, hp.HOP_WILSON)
+    t_sep = lat.timer_stop() / 10
+    sites = sum(int(np.prod(Ls)) for Ls in shapes) * 12
+    emit({"suite": "solver", "case": "ragged_batch_f32_l12_3lattices_12rhs", "us_one_launch": round(t_batch * 1e3, 1),
+          "us_separate_launches": round(t_sep * 1e3, 1), "gflops_one_launch": round(FLOP_PER_SITE * sites / t_batch / 1e6, 1)})
+    for k in range(n):
+        fis[k].close(); fos[k].close(); gs_[k].close(); lats[k].close()
+    # autotuner at 32^3 x 64
+    for prec, links, R in ((0, 18, 1), (1, 12, 12), (2, 12, 12)):
+        g = hp.Gauge(lat, prec, links).upload(u)
+        fi, fo = hp.Field(lat, prec, R), hp.Field(lat, prec, R)
+        for r in range(R):
+            fi.upload(hp.random_spinor(V, 10 + r), r)
+        t_def = time_hop(hp, g, fi, fo)
+        best = (C.c_int * 4)()
+        bms = C.c_double()
+        capi.check(lib.plqcd_hip_autotune(g.h, fi.h, fo.h, 5, best, C.byref(bms)), "autotune")
+        t_tuned = time_hop(hp, g, fi, fo)
+        emit({"suite": "solver", "case": f"autotune_32c64_{PREC_NAMES[prec]}_l{links}_R{R}",
+              "default_us": round(t_def * 1e3, 1), "tuned_us": round(t_tuned * 1e3, 1),
+              "best": {"order": best[0], "group": best[1], "block": best[2], "tmarch": best[3]}})
+        fi.close(); fo.close(); g.close()
+    lib.plqcd_hip_autotune_clear()
+    lat.close()
+
+
+def calib_suite():
+    """HBM probes of known byte counts (copy 1 GiB, read 1 GiB, write 1 GiB),
+    then two applications of H per storage format at 32^3 x 64 with the default
+    launch shapes: one run holds the counter calibration and the kernels."""
+    import ctypes as C
+    from plqcd_amd import capi, hopping as hp
+    lib = capi.lib()
+    for kind, name in ((0, "copy"), (1, "read"), (2, "write")):
+        gbs = C.c_double()
+        capi.check(lib.plqcd_hip_stream_bandwidth(kind, 1 << 30, 1, C.byref(gbs)), "stream_bandwidth")
+        emit({"suite": "calib", "case": f"stream_{name}_1GiB", "bytes_per_buffer": 1 << 30, "GBps": round(gbs.value, 1)})
+    L = (32, 32, 32, 64)
+    V = int(np.prod(L))
+    u = hp.random_gauge_su3(V, 1)
+    q = hp.random_spinor(V, 2)
+    lat = hp.Lattice(L)
+    for prec, links in ((0, 18), (1, 18), (1, 12), (2, 12)):
+        g = hp.Gauge(lat, prec, links).upload(u)
+        fi, fo = hp.Field(lat, prec).upload(q), hp.Field(lat, prec)
+        for _ in range(2):
+            hp.hop(g, fi, fo, hp.PARITY_ALL, hp.HOP_WILSON)
+        lat.sync()
+        emit({"suite": "calib", "case": f"hop_{PREC_NAMES[prec]}_l{links}_32x32x32x64", "applications": 2,
+              "compulsory_bytes": BYTES_PER_SITE[(prec, links)] * V})
+        fi.close(); fo.close(); g.close()
+    lat.close()
+
+
+def serve_suite(duration: float, rate: float, seed: int = 7):
+    """Poisson request stream for `duration` seconds; volumes from
+    {16^4, 24^3x48, 32^3x64, 48^3x96}, 1..12 RHS, tol in {1e-8,1e-10,1e-12},
+    kappa in [0.120, 0.1245], two gauge fields per volume.  Reports p50/p99
+    latency and solves/s beside one-at-a-time plain FP64 solves."""
+    from plqcd_amd import hopping as hp, solver as sv
+    from plqcd_amd.serve import Server
+    rng = np.random.default_rng(seed)
+    vols = [(16, 16, 16, 16), (24, 24, 24, 48), (32, 32, 32, 64), (48, 48, 48, 96)]
+    pool = {L: np.stack([hp.random_spinor(int(np.prod(L)), 1000 + k) for k in range(12)]) for L in vols}
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=int(rate * duration * 2) + 10))
+    arrivals = arrivals[arrivals < duration]
+    reqs = []
+    for t in arrivals:
+        L = vols[rng.integers(len(vols))]
+        reqs.append({"t": float(t), "L": L, "nrhs": int(rng.integers(1, 13)), "tol": float(rng.choice([1e-8, 1e-10, 1e-12])),
+                     "kappa": float(rng.uniform(0.120, 0.1245)), "seed": 11 + 100 * vols.index(L) + int(rng.integers(2))})
+    lat_res = {}
+    with Server(sloppy_prec=2, sloppy_links=12, slots=12) as srv:
+        t0 = time.perf_counter()
+        submitted = {}
+        k = 0
+        done = {}
+        while k < len(reqs) or len(done) < len(reqs):
+            now = time.perf_counter() - t0
+            while k < len(reqs) and reqs[k]["t"] <= now:
+                r = reqs[k]
+                rid = srv.submit(r["L"], r["seed"], pool[r["L"]][: r["nrhs"]], r["kappa"], r["tol"])
+                submitted[rid] = r
+                k += 1
+            for d in srv.poll():
+                done[d["id"]] = d
+                d.pop("x", None)
+            time.sleep(0.002)
+        wall = time.perf_counter() - t0
+        info = srv.info()
+    lat_s = np.array([d["end_time"] - d["submit_time"] for d in done.values()])
+    nrhs = sum(submitted[i]["nrhs"] for i in done)
+    ok = all(d["status"] == 0 and d["converged"] for d in done.values())
+    # one-at-a-time plain FP64 baseline: seconds per single-RHS solve, per volume
+    per_rhs = {}
+    for L in vols:
+        V = int(np.prod(L))
+        la = hp.Lattice(L)
+        g = hp.Gauge(la).upload(hp.random_gauge_su3(V, 11 + 100 * vols.index(L)))
+        b = hp.Field(la).upload(pool[L][0])
+        x = hp.Field(la)
+        sv.cg_solve(g, b, x, 0.1225, 1e-10)
+        t1 = time.perf_counter()
+        sv.cg_solve(g, b, x, 0.1225, 1e-10)
+        per_rhs[L] = time.perf_counter() - t1
+        x.close(); b.close(); g.close(); la.close()
+    base_seconds = sum(per_rhs[submitted[i]["L"]] * submitted[i]["nrhs"] for i in done)
+    emit({"suite": "serve", "case": "poisson_mixed_volumes", "duration_s": duration, "rate_per_s": rate,
+          "requests": len(done), "rhs_solved": nrhs, "all_converged": ok, "wall_s": round(wall, 2),
+          "p50_latency_s": round(float(np.percentile(lat_s, 50)), 4),
+          "p99_latency_s": round(float(np.percentile(lat_s, 99)), 4),
+          "solves_per_s": round(nrhs / wall, 3),
+          "fp64_one_at_a_time_solves_per_s": round(nrhs / base_seconds, 3),
+          "fp64_seconds_per_rhs": {"x".join(map(str, L)): round(v, 4) for L, v in per_rhs.items()},
+          "hbm_budget_gb": round(info["budget"] / 2**30, 1), "failed": info["failed"]})
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -120,11 +360,26 @@ def main():
     ap.add_argument("--mode", choices=["wilson", "compat"], default="wilson")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-solver", action="store_true")
+    ap.add_argument("--suite", choices=["hop", "solver", "serve", "calib"], default="hop",
+                    help="hop: the headline line (default); solver/serve: the extended studies, one JSON line each")
+    ap.add_argument("--duration", type=float, default=60.0, help="serve suite: seconds of Poisson arrivals")
+    ap.add_argument("--rate", type=float, default=2.0, help="serve suite: requests per second")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): the hopping matrix here is a "
               "single-GPU kernel; multi-GPU sharding is a design, not a build", file=sys.stderr)
         sys.exit(2)
+
+    if args.suite == "solver":
+        solver_suite()
+        return
+    if args.suite == "serve":
+        serve_suite(args.duration, args.rate)
+        return
+    if args.suite == "calib":
+        calib_suite()
+        return
 
     L = tuple(int(x) for x in args.lattice.split(","))
     V = int(np.prod(L))
@@ -159,6 +414,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(tag)},
         "cpu_baseline": cpu,
+        "solver": None if args.no_solver else solve_compare(L),
     }
     print(json.dumps(rec), flush=True)
 

@@ -207,6 +207,14 @@ int plqcd_hip_hop(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip
 int plqcd_hip_apply_m(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip_field *out,
                       const double *kappa_rhs, int dagger);
 
+/* Even-odd (Schur) preconditioned operator on one parity p (EVEN or ODD):
+ * out_p = in_p - kappa_r^2 H_{p,q} H_{q,p} in_p  (q the other parity), built
+ * from the eo/oe halves of dirac.h:21-22 in wilson mode.  tmp receives
+ * H_{q,p} in_p on parity q; the other parity of out is not touched.
+ * Synchronous. */
+int plqcd_hip_apply_schur(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip_field *tmp,
+                          plqcd_hip_field *out, const double *kappa_rhs, int parity);
+
 typedef struct {
    double kappa;            /* M = 1 - kappa H                                  */
    double tol;              /* stop at ||b - M^+M x|| / ||b|| <= tol (FP64)     */
@@ -236,6 +244,56 @@ typedef struct {
 int plqcd_hip_cg_solve(const plqcd_hip_gauge *g, const plqcd_hip_gauge *g_sloppy, const plqcd_hip_field *b,
                        plqcd_hip_field *x, const plqcd_cg_params *params, plqcd_cg_stats *stats);
 
+/* ------------------------------------------------------------- serving */
+/* A solve server with continuous batching: requests for CG solves of
+ * M^+M x = b on any lattice are admitted by free HBM, share resident gauge
+ * fields, and join running multi-RHS solves as slots free up.  New; the
+ * reference has no serving layer. */
+typedef struct plqcd_hip_server plqcd_hip_server;
+
+typedef struct {
+   int device;         /* HIP device                                            */
+   int sloppy_prec;    /* inner CG precision: PLQCD_PREC_F64 / F32 / F16         */
+   int sloppy_links;   /* 18 or 12 (F16: 12)                                     */
+   int slots;          /* right-hand sides per engine, 1..32                     */
+   size_t hbm_budget;  /* bytes the server may allocate (0: free HBM - 4 GiB)    */
+   int use_graph;      /* replay captured CG iterations                          */
+} plqcd_serve_config;
+
+typedef struct {
+   int L[4];             /* lattice extents                                     */
+   uint64_t gauge_seed;  /* names the gauge field; generated as random SU(3)
+                            with this seed unless `gauge` is given              */
+   const su3 *gauge;     /* optional links (4V, reference layout) for the first
+                            request naming this (L, seed)                       */
+   double kappa, tol;
+   int max_iter;
+   int nrhs;
+   const spinor *b;      /* nrhs*V sources; must stay valid until completion   */
+   spinor *x;            /* nrhs*V solutions, written before completion        */
+} plqcd_serve_request;
+
+typedef struct {
+   uint64_t id;
+   int status;            /* 0 ok, -1 device error, -2 larger than the budget  */
+   double submit_time, start_time, end_time;  /* seconds since server creation */
+   int iterations;        /* largest over the request's RHS                    */
+   double true_residual;  /* largest FP64 true relative residual               */
+   int converged;         /* all RHS converged                                 */
+} plqcd_serve_result;
+
+int plqcd_hip_server_create(const plqcd_serve_config *cfg, plqcd_hip_server **out);
+/* Thread-safe.  Returns at once; the request is served in the background. */
+int plqcd_hip_server_submit(plqcd_hip_server *s, const plqcd_serve_request *req, uint64_t *id);
+/* Moves up to `max` completed results into out[]; *n = how many. */
+int plqcd_hip_server_poll(plqcd_hip_server *s, plqcd_serve_result *out, int max, int *n);
+/* Waits until at least `count` results are ready to poll. */
+int plqcd_hip_server_wait(plqcd_hip_server *s, int count, double timeout_s);
+/* out = {budget bytes, bytes in use, cached (lattice, gauge) entries, internal failures} */
+int plqcd_hip_server_info(plqcd_hip_server *s, double out[4]);
+/* Finishes the queued requests, then frees everything. */
+void plqcd_hip_server_destroy(plqcd_hip_server *s);
+
 /* Launch shape of the hopping kernels for storage precision `prec` (-1: all
  * precisions), process-wide; the autotuner sets it.
  *   order 0: block b -> tile b; 1: each XCD takes a contiguous eighth of the
@@ -249,6 +307,27 @@ int plqcd_hip_cg_solve(const plqcd_hip_gauge *g, const plqcd_hip_gauge *g_sloppy
 int plqcd_hip_hop_tuning(int prec, int order, int group, int block, int tmarch);
 /* Current launch shape of precision `prec`: out = {order, group, block, tmarch}. */
 int plqcd_hip_hop_get_tuning(int prec, int out[4]);
+/* Times every launch shape for the shape of (g, in) -- lattice, precision,
+ * link format, RHS count -- with `reps` timed applications each, and makes
+ * the fastest the launch shape of that shape from now on.  best = {order,
+ * group, block, tmarch}, *best_ms = its time per application.  in/out are
+ * overwritten. */
+int plqcd_hip_autotune(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip_field *out, int reps,
+                       int best[4], double *best_ms);
+/* Forgets every autotuned shape (back to the per-precision defaults). */
+int plqcd_hip_autotune_clear(void);
+
+/* One application of H (full lattice, given mode) on each of n independent
+ * lattices -- each with its own gauge field and fields, any volume, up to 32
+ * RHS -- in a single persistent launch.  All items share storage precision,
+ * link format and stream.  Asynchronous. */
+int plqcd_hip_hop_batch(int n, const plqcd_hip_gauge *const *g, const plqcd_hip_field *const *in,
+                        plqcd_hip_field *const *out, int mode);
+
+/* HBM probe: achieved GB/s of a streaming kernel over `bytes` per buffer,
+ * best of `reps` back-to-back launches timed together.  kind 0: copy (reads
+ * and writes `bytes`); 1: read only; 2: write only. */
+int plqcd_hip_stream_bandwidth(int kind, size_t bytes, int reps, double *gbs);
 
 /* ------------------------------------------------ synthetic inputs */
 /* Seeded, counter-based and thread-order independent: the value at each

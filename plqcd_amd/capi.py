@@ -32,6 +32,22 @@ class CgStats(C.Structure):
                 ("true_residual", C.c_double), ("seconds", C.c_double), ("hop_applications", C.c_double)]
 
 
+class ServeConfig(C.Structure):
+    _fields_ = [("device", C.c_int), ("sloppy_prec", C.c_int), ("sloppy_links", C.c_int), ("slots", C.c_int),
+                ("hbm_budget", C.c_size_t), ("use_graph", C.c_int)]
+
+
+class ServeRequest(C.Structure):
+    _fields_ = [("L", C.c_int * 4), ("gauge_seed", C.c_uint64), ("gauge", C.c_void_p), ("kappa", C.c_double),
+                ("tol", C.c_double), ("max_iter", C.c_int), ("nrhs", C.c_int), ("b", C.c_void_p), ("x", C.c_void_p)]
+
+
+class ServeResult(C.Structure):
+    _fields_ = [("id", C.c_uint64), ("status", C.c_int), ("submit_time", C.c_double), ("start_time", C.c_double),
+                ("end_time", C.c_double), ("iterations", C.c_int), ("true_residual", C.c_double),
+                ("converged", C.c_int)]
+
+
 # name -> (restype, argtypes)
 _PROTOS = {
     "init_plqcd": (C.c_int, [C.c_int, C.POINTER(C.c_char_p)]),
@@ -74,9 +90,20 @@ _PROTOS = {
     "plqcd_hip_field_bytes": (C.c_size_t, [vp]),
     "plqcd_hip_hop": (C.c_int, [vp, vp, vp, C.c_int, C.c_int]),
     "plqcd_hip_apply_m": (C.c_int, [vp, vp, vp, C.POINTER(C.c_double), C.c_int]),
+    "plqcd_hip_apply_schur": (C.c_int, [vp, vp, vp, vp, C.POINTER(C.c_double), C.c_int]),
     "plqcd_hip_cg_solve": (C.c_int, [vp, vp, vp, vp, C.POINTER(CgParams), C.POINTER(CgStats)]),
+    "plqcd_hip_server_create": (C.c_int, [C.POINTER(ServeConfig), C.POINTER(vp)]),
+    "plqcd_hip_server_submit": (C.c_int, [vp, C.POINTER(ServeRequest), C.POINTER(C.c_uint64)]),
+    "plqcd_hip_server_poll": (C.c_int, [vp, C.POINTER(ServeResult), C.c_int, C.POINTER(C.c_int)]),
+    "plqcd_hip_server_wait": (C.c_int, [vp, C.c_int, C.c_double]),
+    "plqcd_hip_server_info": (C.c_int, [vp, C.POINTER(C.c_double)]),
+    "plqcd_hip_server_destroy": (None, [vp]),
+    "plqcd_hip_autotune": (C.c_int, [vp, vp, vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
+    "plqcd_hip_autotune_clear": (C.c_int, []),
+    "plqcd_hip_hop_batch": (C.c_int, [C.c_int, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.c_int]),
     "plqcd_hip_hop_tuning": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
     "plqcd_hip_hop_get_tuning": (C.c_int, [C.c_int, C.POINTER(C.c_int)]),
+    "plqcd_hip_stream_bandwidth": (C.c_int, [C.c_int, C.c_size_t, C.c_int, C.POINTER(C.c_double)]),
     "plqcd_random_gauge_su3": (None, [vp, C.c_int, C.c_uint64]),
     "plqcd_random_spinor_gauss": (None, [vp, C.c_int, C.c_uint64]),
     "plqcd_unit_gauge": (None, [vp, C.c_int]),

@@ -17,14 +17,14 @@ constexpr int kPackBlock = 256;
 
 // aos: n sites starting at even-odd index i0, 12 complex doubles each.
 template <int PREC>
-__global__ void pack_spinor(const d2 *__restrict__ aos, SpinorIO<PREC> f, int rhs, int i0, int n)
+__global__ void pack_spinor(const d2 *__restrict__ aos, SpinorIO<PREC> f, int Vh, int rhs, int i0, int n)
 {
    const int t = blockIdx.x * blockDim.x + threadIdx.x;
    if (t >= n) return;
    f = f.at_rhs(rhs);
    const int i = i0 + t;
-   const int p = i >= f.Vh;
-   const int j = i - p * f.Vh;
+   const int p = i >= Vh;
+   const int j = i - p * Vh;
    typename SpinorIO<PREC>::C s[12];
 #pragma unroll
    for (int c = 0; c < 12; c++) {
@@ -36,14 +36,14 @@ __global__ void pack_spinor(const d2 *__restrict__ aos, SpinorIO<PREC> f, int rh
 }
 
 template <int PREC>
-__global__ void unpack_spinor(d2 *__restrict__ aos, SpinorIO<PREC> f, int rhs, int i0, int n)
+__global__ void unpack_spinor(d2 *__restrict__ aos, SpinorIO<PREC> f, int Vh, int rhs, int i0, int n)
 {
    const int t = blockIdx.x * blockDim.x + threadIdx.x;
    if (t >= n) return;
    f = f.at_rhs(rhs);
    const int i = i0 + t;
-   const int p = i >= f.Vh;
-   const int j = i - p * f.Vh;
+   const int p = i >= Vh;
+   const int j = i - p * Vh;
    typename SpinorIO<PREC>::C s[12];
    f.load(p, j, s);
 #pragma unroll
@@ -291,15 +291,15 @@ int plqcd::field_upload_range(plqcd_hip_field *f, int rhs, const spinor *host, i
       switch (f->prec) {
          case PLQCD_PREC_F64:
             hipLaunchKernelGGL(pack_spinor<PLQCD_PREC_F64>, grid, block, 0, lat->stream, src,
-                               spinor_io<PLQCD_PREC_F64>(f), rhs, i0, n);
+                               spinor_io<PLQCD_PREC_F64>(f), lat->Vh, rhs, i0, n);
             break;
          case PLQCD_PREC_F32:
             hipLaunchKernelGGL(pack_spinor<PLQCD_PREC_F32>, grid, block, 0, lat->stream, src,
-                               spinor_io<PLQCD_PREC_F32>(f), rhs, i0, n);
+                               spinor_io<PLQCD_PREC_F32>(f), lat->Vh, rhs, i0, n);
             break;
          default:
             hipLaunchKernelGGL(pack_spinor<PLQCD_PREC_F16>, grid, block, 0, lat->stream, src,
-                               spinor_io<PLQCD_PREC_F16>(f), rhs, i0, n);
+                               spinor_io<PLQCD_PREC_F16>(f), lat->Vh, rhs, i0, n);
       }
       PLQCD_HIP_TRY(hipGetLastError());
    }
@@ -322,15 +322,15 @@ int plqcd::field_download_range(const plqcd_hip_field *f, int rhs, spinor *host,
       switch (f->prec) {
          case PLQCD_PREC_F64:
             hipLaunchKernelGGL(unpack_spinor<PLQCD_PREC_F64>, grid, block, 0, lat->stream, dst,
-                               spinor_io<PLQCD_PREC_F64>(f), rhs, i0, n);
+                               spinor_io<PLQCD_PREC_F64>(f), lat->Vh, rhs, i0, n);
             break;
          case PLQCD_PREC_F32:
             hipLaunchKernelGGL(unpack_spinor<PLQCD_PREC_F32>, grid, block, 0, lat->stream, dst,
-                               spinor_io<PLQCD_PREC_F32>(f), rhs, i0, n);
+                               spinor_io<PLQCD_PREC_F32>(f), lat->Vh, rhs, i0, n);
             break;
          default:
             hipLaunchKernelGGL(unpack_spinor<PLQCD_PREC_F16>, grid, block, 0, lat->stream, dst,
-                               spinor_io<PLQCD_PREC_F16>(f), rhs, i0, n);
+                               spinor_io<PLQCD_PREC_F16>(f), lat->Vh, rhs, i0, n);
       }
       PLQCD_HIP_TRY(hipGetLastError());
       PLQCD_HIP_TRY(

@@ -45,4 +45,21 @@ struct DslashOp {
 int dslash(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip_field *out, int parity, int mode,
            const DslashOp &op);
 
+// Launch shapes found by the autotuner, per (lattice, precision, links, RHS).
+struct TuneKey {
+   int L[4];
+   int prec, links, R;
+   bool operator<(const TuneKey &o) const
+   {
+      for (int i = 0; i < 4; i++)
+         if (L[i] != o.L[i]) return L[i] < o.L[i];
+      if (prec != o.prec) return prec < o.prec;
+      if (links != o.links) return links < o.links;
+      return R < o.R;
+   }
+};
+bool lookup_tuning(const TuneKey &k, HopTuning *t);
+void store_tuning(const TuneKey &k, const HopTuning &t);
+void clear_tuning(const TuneKey *k);  // null: all
+
 }  // namespace plqcd

@@ -24,6 +24,11 @@
 // do not reach s2/s3 (dirac.c:373-374 and :756-757 assign the mu=0 backward
 // term over them) and odd output sites take their forward links from the even
 // site with the same index inside its parity (dirac.c:643-644).
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <vector>
+
 #include "fields.h"
 #include "dslash.h"
 #include "reduce.h"
@@ -248,11 +253,12 @@ __device__ __forceinline__ double epilogue(const SIO &in, const SIO &out, const 
    return nrm;
 }
 
+// One block of the gather kernel: logical block `blk` of the launch.
 template <int MODE, bool DAG, int EPI, bool NORM, int BLOCK, class SIO, class GIO>
-__global__ __launch_bounds__(BLOCK) void hop_kernel(SIO in, SIO out, GIO g, HopArgs a)
+__device__ __forceinline__ void hop_block(const SIO &in, const SIO &out, const GIO &g, const HopArgs &a, int blk)
 {
    typedef typename SIO::C C;
-   int p, blk = logical_block(a);
+   int p;
    if (a.nparity == 2) {
       p = blk & 1;
       blk >>= 1;
@@ -263,7 +269,7 @@ __global__ __launch_bounds__(BLOCK) void hop_kernel(SIO in, SIO out, GIO g, HopA
    const int js = threadIdx.x / a.R;
    const int j = blk * a.spb + js;
    double nrm = 0.0;
-   if (j < a.Vh) {
+   if (js < a.spb && j < a.Vh) {
       const SIO si = in.at_rhs(r), so = out.at_rhs(r);
       const Nbr n = neighbours(a, p, j);
       C acc[12];
@@ -277,6 +283,37 @@ __global__ __launch_bounds__(BLOCK) void hop_kernel(SIO in, SIO out, GIO g, HopA
    }
    if (NORM) chunk_reduce<BLOCK>(nrm, js, r, a.R, p * a.nchunk + (blk * a.spb + js) / REDUCE_CHUNK,
                                  j - (js & (REDUCE_CHUNK - 1)) < a.Vh, a.partial + (size_t)r * 2 * a.nchunk);
+}
+
+template <int MODE, bool DAG, int EPI, bool NORM, int BLOCK, class SIO, class GIO>
+__global__ __launch_bounds__(BLOCK) void hop_kernel(SIO in, SIO out, GIO g, HopArgs a)
+{
+   hop_block<MODE, DAG, EPI, NORM, BLOCK>(in, out, g, a, logical_block(a));
+}
+
+// One application of H on each of several independent lattices (each with its
+// own gauge field and fields) in a single persistent launch: the grid holds as
+// many blocks as are resident at once, and block b walks the concatenated
+// logical blocks b, b + gridDim.x, ...  (every block leaves after a bounded
+// number of iterations).
+template <class SIO, class GIO>
+struct BatchItem {
+   SIO in, out;
+   GIO g;
+   HopArgs a;
+   int first;  // first global block of this item
+   int nb;     // blocks of this item
+};
+
+template <int MODE, int BLOCK, class SIO, class GIO>
+__global__ __launch_bounds__(BLOCK) void hop_batch_kernel(const BatchItem<SIO, GIO> *items, int nitems, int total)
+{
+   for (int b = blockIdx.x; b < total; b += gridDim.x) {
+      int it = 0;
+      while (it + 1 < nitems && b >= items[it + 1].first) it++;
+      const BatchItem<SIO, GIO> &I = items[it];
+      hop_block<MODE, false, 0, false, BLOCK>(I.in, I.out, I.g, I.a, b - I.first);
+   }
 }
 
 // t-march variant (wilson mode, full lattice): a thread owns one (x,y,z)
@@ -407,11 +444,10 @@ int launch_t(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip_fiel
    typedef SpinorIO<PREC> S;
    typedef GaugeIO<PREC, LINKS> G;
    const S si = spinor_io<PREC>(in), so = spinor_io<PREC>(out);
-   G gg;
-   gg.base = reinterpret_cast<decltype(gg.base)>(g->data);
-   gg.Vh = lat->Vh;
+   const G gg = gauge_io<PREC, LINKS>(g);
    const int R = in->nrhs;
-   const HopTuning &tu = g_tuning[PREC];
+   HopTuning tu = g_tuning[PREC];
+   plqcd::lookup_tuning(plqcd::TuneKey{{lat->L[0], lat->L[1], lat->L[2], lat->L[3]}, PREC, LINKS, in->nrhs}, &tu);
    // block shape: R lanes per site and a whole number of reduction chunks
    int target = tu.block;
    if (R * REDUCE_CHUNK > target) {
@@ -467,6 +503,108 @@ int launch_t(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip_fiel
 
 // Defaults per storage precision (F64, F32, F16), from sweeps at 32^3 x 64.
 HopTuning g_tuning[3] = {{1, 1, 256, 16}, {1, 1, 256, 16}, {1, 1, 64, 4}};
+
+namespace {
+std::mutex g_tune_mu;
+std::map<plqcd::TuneKey, HopTuning> g_tune_cache;
+}  // namespace
+
+bool plqcd::lookup_tuning(const TuneKey &k, HopTuning *t)
+{
+   std::lock_guard<std::mutex> lk(g_tune_mu);
+   auto it = g_tune_cache.find(k);
+   if (it == g_tune_cache.end()) return false;
+   *t = it->second;
+   return true;
+}
+
+void plqcd::store_tuning(const TuneKey &k, const HopTuning &t)
+{
+   std::lock_guard<std::mutex> lk(g_tune_mu);
+   g_tune_cache[k] = t;
+}
+
+void plqcd::clear_tuning(const TuneKey *k)
+{
+   std::lock_guard<std::mutex> lk(g_tune_mu);
+   if (k) g_tune_cache.erase(*k);
+   else g_tune_cache.clear();
+}
+
+namespace {
+
+template <int PREC, int LINKS>
+int launch_batch_t(int n, const plqcd_hip_gauge *const *g, const plqcd_hip_field *const *in,
+                   plqcd_hip_field *const *out, int mode)
+{
+   typedef SpinorIO<PREC> S;
+   typedef GaugeIO<PREC, LINKS> G;
+   std::vector<BatchItem<S, G>> items(n);
+   int total = 0, maxR = 1;
+   for (int i = 0; i < n; i++) maxR = std::max(maxR, in[i]->nrhs);
+   if (maxR * REDUCE_CHUNK > 256) return plqcd::set_error(2, "hop_batch: at most 32 right-hand sides per item");
+   for (int i = 0; i < n; i++) {
+      BatchItem<S, G> &I = items[i];
+      I.in = spinor_io<PREC>(in[i]);
+      I.out = spinor_io<PREC>(out[i]);
+      I.g = gauge_io<PREC, LINKS>(g[i]);
+      I.a = make_hop_args(g[i]->lat, PLQCD_PARITY_ALL);
+      I.a.R = in[i]->nrhs;
+      I.a.spb = REDUCE_CHUNK * (256 / (REDUCE_CHUNK * I.a.R));
+      I.nb = 2 * ((I.a.Vh + I.a.spb - 1) / I.a.spb);
+      I.first = total;
+      total += I.nb;
+   }
+   hipStream_t st = g[0]->lat->stream;
+   BatchItem<S, G> *d = nullptr;
+   PLQCD_HIP_TRY(hipMallocAsync((void **)&d, n * sizeof(BatchItem<S, G>), st));
+   hipError_t e = hipMemcpyAsync(d, items.data(), n * sizeof(BatchItem<S, G>), hipMemcpyHostToDevice, st);
+   int rc = 0;
+   if (e == hipSuccess) {
+      int dev = 0, ncu = 0, per = 0;
+      (void)hipGetDevice(&dev);
+      (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+      if (mode == PLQCD_HOP_COMPAT) {
+         (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, hop_batch_kernel<PLQCD_HOP_COMPAT, 256, S, G>, 256, 0);
+         const int grid = std::max(1, std::min(total, std::max(per, 1) * ncu));
+         hipLaunchKernelGGL((hop_batch_kernel<PLQCD_HOP_COMPAT, 256, S, G>), dim3(grid), dim3(256), 0, st, d, n, total);
+      } else {
+         (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, hop_batch_kernel<PLQCD_HOP_WILSON, 256, S, G>, 256, 0);
+         const int grid = std::max(1, std::min(total, std::max(per, 1) * ncu));
+         hipLaunchKernelGGL((hop_batch_kernel<PLQCD_HOP_WILSON, 256, S, G>), dim3(grid), dim3(256), 0, st, d, n, total);
+      }
+      e = hipGetLastError();
+   }
+   if (e != hipSuccess) rc = plqcd::set_error(10, "hop_batch: %s", hipGetErrorString(e));
+   (void)hipFreeAsync(d, st);
+   return rc;
+}
+
+}  // namespace
+
+extern "C" int plqcd_hip_hop_batch(int n, const plqcd_hip_gauge *const *g, const plqcd_hip_field *const *in,
+                                   plqcd_hip_field *const *out, int mode)
+{
+   plqcd::clear_error();
+   if (n < 1 || !g || !in || !out) return plqcd::set_error(1, "hop_batch: bad arguments");
+   for (int i = 0; i < n; i++) {
+      if (!g[i] || !in[i] || !out[i] || in[i] == out[i] || in[i]->lat != g[i]->lat || out[i]->lat != g[i]->lat ||
+          in[i]->nrhs != out[i]->nrhs || in[i]->prec != g[i]->prec || out[i]->prec != g[i]->prec ||
+          g[i]->prec != g[0]->prec || g[i]->links != g[0]->links)
+         return plqcd::set_error(1, "hop_batch: item %d is inconsistent (all items share precision and link format)", i);
+      if (g[i]->lat->stream != g[0]->lat->stream)
+         return plqcd::set_error(1, "hop_batch: all lattices must use the same stream (set_stream)");
+   }
+   if (mode != PLQCD_HOP_COMPAT && mode != PLQCD_HOP_WILSON) return plqcd::set_error(1, "hop_batch: bad mode");
+   switch (g[0]->prec * 100 + g[0]->links) {
+      case PLQCD_PREC_F64 * 100 + 18: return launch_batch_t<PLQCD_PREC_F64, 18>(n, g, in, out, mode);
+      case PLQCD_PREC_F64 * 100 + 12: return launch_batch_t<PLQCD_PREC_F64, 12>(n, g, in, out, mode);
+      case PLQCD_PREC_F32 * 100 + 18: return launch_batch_t<PLQCD_PREC_F32, 18>(n, g, in, out, mode);
+      case PLQCD_PREC_F32 * 100 + 12: return launch_batch_t<PLQCD_PREC_F32, 12>(n, g, in, out, mode);
+      case PLQCD_PREC_F16 * 100 + 12: return launch_batch_t<PLQCD_PREC_F16, 12>(n, g, in, out, mode);
+      default: return plqcd::set_error(1, "hop_batch: unsupported gauge format");
+   }
+}
 
 HopArgs make_hop_args(const plqcd_hip_lattice *lat, int parity)
 {

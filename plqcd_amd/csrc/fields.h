@@ -15,6 +15,24 @@
 // link the request is served once.  With R = 1 this is plain
 // structure-of-arrays.  An accessor is bound to one RHS by at_rhs(r).
 
+// Addressing: the wave-uniform part of an address (field base, parity,
+// component, direction) is formed in scalar registers; each lane adds one
+// 32-bit byte offset per site, (j*R + r)*16.  The backend then issues
+// global loads in the saddr + 32-bit voffset form with no per-component
+// vector arithmetic.  (A parity block of one component of a field must stay
+// below 4 GiB: Vh*R*16 bytes.)
+
+template <class T>
+__device__ __forceinline__ T ld(const char *ubase, uint32_t off)
+{
+   return *reinterpret_cast<const T *>(ubase + off);
+}
+template <class T>
+__device__ __forceinline__ void st_nt(char *ubase, uint32_t off, T v)
+{
+   __builtin_nontemporal_store(v, reinterpret_cast<T *>(ubase + off));
+}
+
 template <int PREC>
 struct SpinorIO;
 
@@ -23,27 +41,32 @@ struct SpinorIO<PLQCD_PREC_F64> {
    typedef double real;
    typedef d2 C;
    static constexpr int NC = 12;
-   d2 *base;
-   int Vh, R;
-   __device__ __forceinline__ size_t idx(int parity, int c, int j) const
+   char *base;        // field base
+   size_t cstride;    // bytes between components (Vh*R*16)
+   uint32_t R16;      // R*16: bytes between sites
+   uint32_t roff;     // r*16 for the lane's RHS
+   __device__ __forceinline__ uint32_t off(int j) const { return (uint32_t)j * R16 + roff; }
+   __device__ __forceinline__ const char *cbase(int parity, int c) const
    {
-      return ((size_t)(parity * NC + c) * Vh + j) * R;
+      return base + (size_t)(parity * NC + c) * cstride;
    }
    __device__ __forceinline__ SpinorIO at_rhs(int r) const
    {
       SpinorIO s = *this;
-      s.base += r;
+      s.roff = (uint32_t)r * 16;
       return s;
    }
    __device__ __forceinline__ void load(int parity, int j, C s[12]) const
    {
+      const uint32_t o = off(j);
 #pragma unroll
-      for (int c = 0; c < 12; c++) s[c] = base[idx(parity, c, j)];
+      for (int c = 0; c < 12; c++) s[c] = ld<d2>(cbase(parity, c), o);
    }
    __device__ __forceinline__ void store(int parity, int j, const C s[12]) const
    {
+      const uint32_t o = off(j);
 #pragma unroll
-      for (int c = 0; c < 12; c++) __builtin_nontemporal_store(s[c], base + idx(parity, c, j));
+      for (int c = 0; c < 12; c++) st_nt(const_cast<char *>(cbase(parity, c)), o, s[c]);
    }
 };
 
@@ -52,33 +75,36 @@ struct SpinorIO<PLQCD_PREC_F32> {
    typedef float real;
    typedef f2 C;
    static constexpr int NC = 6;
-   f4 *base;
-   int Vh, R;
-   __device__ __forceinline__ size_t idx(int parity, int c, int j) const
+   char *base;
+   size_t cstride;
+   uint32_t R16, roff;
+   __device__ __forceinline__ uint32_t off(int j) const { return (uint32_t)j * R16 + roff; }
+   __device__ __forceinline__ const char *cbase(int parity, int c) const
    {
-      return ((size_t)(parity * NC + c) * Vh + j) * R;
+      return base + (size_t)(parity * NC + c) * cstride;
    }
    __device__ __forceinline__ SpinorIO at_rhs(int r) const
    {
       SpinorIO s = *this;
-      s.base += r;
+      s.roff = (uint32_t)r * 16;
       return s;
    }
    __device__ __forceinline__ void load(int parity, int j, C s[12]) const
    {
+      const uint32_t o = off(j);
 #pragma unroll
       for (int c = 0; c < 6; c++) {
-         const f4 v = base[idx(parity, c, j)];
+         const f4 v = ld<f4>(cbase(parity, c), o);
          s[2 * c] = f2{v.x, v.y};
          s[2 * c + 1] = f2{v.z, v.w};
       }
    }
    __device__ __forceinline__ void store(int parity, int j, const C s[12]) const
    {
+      const uint32_t o = off(j);
 #pragma unroll
       for (int c = 0; c < 6; c++)
-         __builtin_nontemporal_store(f4{s[2 * c].x, s[2 * c].y, s[2 * c + 1].x, s[2 * c + 1].y},
-                                     base + idx(parity, c, j));
+         st_nt(const_cast<char *>(cbase(parity, c)), o, f4{s[2 * c].x, s[2 * c].y, s[2 * c + 1].x, s[2 * c + 1].y});
    }
 };
 
@@ -90,27 +116,31 @@ struct SpinorIO<PLQCD_PREC_F16> {
    typedef float real;
    typedef f2 C;
    static constexpr int NC = 3;
-   u4 *base;
-   float *scale;
-   int Vh, R;
-   __device__ __forceinline__ size_t idx(int parity, int c, int j) const
+   char *base;
+   char *sbase;       // scales
+   size_t cstride;    // bytes between data components (Vh*R*16)
+   size_t spstride;   // bytes between scale parities (Vh*R*4)
+   uint32_t R16, roff, R4, sroff;
+   __device__ __forceinline__ uint32_t off(int j) const { return (uint32_t)j * R16 + roff; }
+   __device__ __forceinline__ uint32_t soff(int j) const { return (uint32_t)j * R4 + sroff; }
+   __device__ __forceinline__ const char *cbase(int parity, int c) const
    {
-      return ((size_t)(parity * NC + c) * Vh + j) * R;
+      return base + (size_t)(parity * NC + c) * cstride;
    }
-   __device__ __forceinline__ size_t sidx(int parity, int j) const { return ((size_t)parity * Vh + j) * R; }
    __device__ __forceinline__ SpinorIO at_rhs(int r) const
    {
       SpinorIO s = *this;
-      s.base += r;
-      s.scale += r;
+      s.roff = (uint32_t)r * 16;
+      s.sroff = (uint32_t)r * 4;
       return s;
    }
    __device__ __forceinline__ void load(int parity, int j, C s[12]) const
    {
-      const float sc = scale[sidx(parity, j)];
+      const uint32_t o = off(j);
+      const float sc = ld<float>(sbase + parity * spstride, soff(j));
 #pragma unroll
       for (int q = 0; q < 3; q++) {
-         const u4 w = base[idx(parity, q, j)];
+         const u4 w = ld<u4>(cbase(parity, q), o);
          s[4 * q + 0] = h2_to_f2(w.x) * sc;
          s[4 * q + 1] = h2_to_f2(w.y) * sc;
          s[4 * q + 2] = h2_to_f2(w.z) * sc;
@@ -123,6 +153,7 @@ struct SpinorIO<PLQCD_PREC_F16> {
 #pragma unroll
       for (int c = 0; c < 12; c++) m = fmaxf(m, fmaxf(fabsf(s[c].x), fabsf(s[c].y)));
       const float inv = m > 0.f ? 1.0f / m : 0.f;
+      const uint32_t o = off(j);
 #pragma unroll
       for (int q = 0; q < 3; q++) {
          u4 w;
@@ -130,9 +161,9 @@ struct SpinorIO<PLQCD_PREC_F16> {
          w.y = f2_to_h2(s[4 * q + 1].x * inv, s[4 * q + 1].y * inv);
          w.z = f2_to_h2(s[4 * q + 2].x * inv, s[4 * q + 2].y * inv);
          w.w = f2_to_h2(s[4 * q + 3].x * inv, s[4 * q + 3].y * inv);
-         __builtin_nontemporal_store(w, base + idx(parity, q, j));
+         st_nt(const_cast<char *>(cbase(parity, q)), o, w);
       }
-      __builtin_nontemporal_store(m, scale + sidx(parity, j));
+      st_nt(sbase + parity * spstride, soff(j), m);
    }
 };
 
@@ -140,12 +171,18 @@ struct SpinorIO<PLQCD_PREC_F16> {
 template <int PREC>
 inline SpinorIO<PREC> spinor_io(const plqcd_hip_field *f)
 {
-   SpinorIO<PREC> s;
-   s.base = reinterpret_cast<decltype(s.base)>(f->data);
-   s.Vh = f->lat->Vh;
-   s.R = f->nrhs;
-   if constexpr (PREC == PLQCD_PREC_F16)
-      s.scale = reinterpret_cast<float *>(static_cast<char *>(f->data) + (size_t)2 * 3 * 16 * s.Vh * s.R);
+   SpinorIO<PREC> s{};
+   const size_t Vh = f->lat->Vh, R = f->nrhs;
+   s.base = static_cast<char *>(f->data);
+   s.cstride = Vh * R * 16;
+   s.R16 = (uint32_t)(R * 16);
+   s.roff = 0;
+   if constexpr (PREC == PLQCD_PREC_F16) {
+      s.sbase = s.base + 2 * 3 * s.cstride;
+      s.spstride = Vh * R * 4;
+      s.R4 = (uint32_t)(R * 4);
+      s.sroff = 0;
+   }
    return s;
 }
 
@@ -157,26 +194,28 @@ struct GaugeIO;
 template <>
 struct GaugeIO<PLQCD_PREC_F64, 18> {
    typedef d2 C;
-   const d2 *base;
-   int Vh;
+   const char *base;
+   size_t kstride;  // Vh*16: bytes between link components
    __device__ __forceinline__ void load(int parity, int mu, int j, C u[9]) const
    {
-      const d2 *p = base + (size_t)(parity * 4 + mu) * 9 * Vh + j;
+      const char *b = base + (size_t)(parity * 4 + mu) * 9 * kstride;
+      const uint32_t o = (uint32_t)j * 16;
 #pragma unroll
-      for (int k = 0; k < 9; k++) u[k] = p[(size_t)k * Vh];
+      for (int k = 0; k < 9; k++) u[k] = ld<d2>(b + k * kstride, o);
    }
 };
 
 template <>
 struct GaugeIO<PLQCD_PREC_F64, 12> {
    typedef d2 C;
-   const d2 *base;
-   int Vh;
+   const char *base;
+   size_t kstride;
    __device__ __forceinline__ void load(int parity, int mu, int j, C u[9]) const
    {
-      const d2 *p = base + (size_t)(parity * 4 + mu) * 6 * Vh + j;
+      const char *b = base + (size_t)(parity * 4 + mu) * 6 * kstride;
+      const uint32_t o = (uint32_t)j * 16;
 #pragma unroll
-      for (int k = 0; k < 6; k++) u[k] = p[(size_t)k * Vh];
+      for (int k = 0; k < 6; k++) u[k] = ld<d2>(b + k * kstride, o);
       reconstruct_row2(u);
    }
 };
@@ -184,27 +223,29 @@ struct GaugeIO<PLQCD_PREC_F64, 12> {
 template <>
 struct GaugeIO<PLQCD_PREC_F32, 18> {
    typedef f2 C;
-   const f2 *base;
-   int Vh;
+   const char *base;
+   size_t kstride;  // Vh*8
    __device__ __forceinline__ void load(int parity, int mu, int j, C u[9]) const
    {
-      const f2 *p = base + (size_t)(parity * 4 + mu) * 9 * Vh + j;
+      const char *b = base + (size_t)(parity * 4 + mu) * 9 * kstride;
+      const uint32_t o = (uint32_t)j * 8;
 #pragma unroll
-      for (int k = 0; k < 9; k++) u[k] = p[(size_t)k * Vh];
+      for (int k = 0; k < 9; k++) u[k] = ld<f2>(b + k * kstride, o);
    }
 };
 
 template <>
 struct GaugeIO<PLQCD_PREC_F32, 12> {
    typedef f2 C;
-   const f4 *base;
-   int Vh;
+   const char *base;
+   size_t kstride;  // Vh*16
    __device__ __forceinline__ void load(int parity, int mu, int j, C u[9]) const
    {
-      const f4 *p = base + (size_t)(parity * 4 + mu) * 3 * Vh + j;
+      const char *b = base + (size_t)(parity * 4 + mu) * 3 * kstride;
+      const uint32_t o = (uint32_t)j * 16;
 #pragma unroll
       for (int k = 0; k < 3; k++) {
-         const f4 v = p[(size_t)k * Vh];
+         const f4 v = ld<f4>(b + k * kstride, o);
          u[2 * k] = f2{v.x, v.y};
          u[2 * k + 1] = f2{v.z, v.w};
       }
@@ -215,13 +256,13 @@ struct GaugeIO<PLQCD_PREC_F32, 12> {
 template <>
 struct GaugeIO<PLQCD_PREC_F16, 12> {
    typedef f2 C;
-   const u4 *base;  // [parity][mu][Vh] u4 (u00,u01,u02,u10), then [parity][mu][Vh] u2 (u11,u12)
-   int Vh;
+   const char *base;  // [parity][mu][Vh] u4 (u00,u01,u02,u10), then [parity][mu][Vh] u2 (u11,u12)
+   size_t kstride;    // Vh*16
    __device__ __forceinline__ void load(int parity, int mu, int j, C u[9]) const
    {
-      const size_t o = (size_t)(parity * 4 + mu) * Vh + j;
-      const u4 a = base[o];
-      const u2 b = reinterpret_cast<const u2 *>(base + (size_t)8 * Vh)[o];
+      const size_t slot = (size_t)(parity * 4 + mu);
+      const u4 a = ld<u4>(base + slot * kstride, (uint32_t)j * 16);
+      const u2 b = ld<u2>(base + 8 * kstride + slot * (kstride / 2), (uint32_t)j * 8);
       u[0] = h2_to_f2(a.x);
       u[1] = h2_to_f2(a.y);
       u[2] = h2_to_f2(a.z);
@@ -231,3 +272,13 @@ struct GaugeIO<PLQCD_PREC_F16, 12> {
       reconstruct_row2(u);
    }
 };
+
+template <int PREC, int LINKS>
+inline GaugeIO<PREC, LINKS> gauge_io(const plqcd_hip_gauge *g)
+{
+   GaugeIO<PREC, LINKS> io;
+   io.base = static_cast<const char *>(g->data);
+   const size_t Vh = g->lat->Vh;
+   io.kstride = (PREC == PLQCD_PREC_F32 && LINKS == 18) ? Vh * 8 : Vh * 16;
+   return io;
+}

@@ -1,165 +1,349 @@
 // Conjugate gradient on the normal equations  M^+ M x = b,  M = 1 - kappa H
-// (wilson mode, M^+ = gamma5 M gamma5), for all right-hand sides of a field
-// at once, with per-RHS kappa and tolerance.
+// (wilson mode, M^+ = gamma5 M gamma5), for a set of right-hand-side slots
+// with per-slot kappa and tolerance.
 //
 // The iteration runs in the "sloppy" precision of the sloppy gauge field
 // (F64, F32 or F16 storage; F32/F64 arithmetic) and is corrected by reliable
 // updates (Sleijpen and van der Vorst 1996, used for mixed precision by Clark
-// et al. 2010): when the iterated residual of a right-hand side has dropped by
-// `delta` since its last update, or looks converged, the partial solution is
-// added to the FP64 solution, the true residual b - M^+M x is recomputed in
-// FP64 and replaces the iterated one.  A right-hand side stops when its FP64
-// true residual meets its tolerance.
+// et al. 2010): when the iterated residual of a slot has dropped by `delta`
+// since its last update, or looks converged, the partial solution is added to
+// the FP64 solution, the true residual b - M^+M x is recomputed in FP64 and
+// replaces the iterated one.  A slot stops when its FP64 true residual meets
+// its tolerance.
 //
 // One iteration is a fixed sequence of kernels (beta, p update, M p with
 // |Mp|^2, M^+, alpha, x/r update with |r|^2) whose scalars live on the
 // device; it is captured once into a hipGraph and replayed.  Only |r|^2 comes
-// back to the host each iteration, to decide reliable updates.  Every
-// per-RHS quantity is computed from that RHS's data alone with fixed-order
-// reductions, so a right-hand side follows bitwise the same iterates whatever
-// else shares the launch.
+// back to the host each iteration, to decide reliable updates.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <new>
 #include <vector>
 
 #include "blas.h"
+#include "cg_engine.h"
 #include "dslash.h"
 #include "reduce.h"
 
 namespace {
 
-constexpr int kMaxRhs = 32;
+constexpr int kMaxSlots = 32;
 
-__global__ void k_alpha(const double *rr, const double *pap, const int *active, double *alpha, int R)
+__global__ void k_alpha(const double *rr, const double *pap, const int *active, double *alpha, int S)
 {
    const int r = threadIdx.x;
-   if (r < R) alpha[r] = (active[r] && pap[r] > 0.0) ? rr[r] / pap[r] : 0.0;
+   if (r < S) alpha[r] = (active[r] && pap[r] > 0.0) ? rr[r] / pap[r] : 0.0;
 }
 
-__global__ void k_beta(double *rr, const double *rr_new, const int *active, double *beta, int R)
+__global__ void k_beta(double *rr, const double *rr_new, const int *active, double *beta, int S)
 {
    const int r = threadIdx.x;
-   if (r < R) {
+   if (r < S) {
       beta[r] = (active[r] && rr[r] > 0.0) ? rr_new[r] / rr[r] : 0.0;
       rr[r] = rr_new[r];
    }
 }
 
-struct Workspace {
-   plqcd_hip_lattice *lat = nullptr;
-   int R = 0;
-   plqcd_hip_field *r = nullptr, *t1 = nullptr, *t2 = nullptr;             // F64
-   plqcd_hip_field *rs = nullptr, *ps = nullptr, *qs = nullptr, *aps = nullptr, *xs = nullptr;  // sloppy
-   double *partial = nullptr;
-   double *scal = nullptr;  // kappa, rr, rr_new, pap, alpha, beta, tmp  (7 x kMaxRhs)
-   int *imask = nullptr;    // active, update  (2 x kMaxRhs)
-   double *host = nullptr;  // pinned: 2 x kMaxRhs
-   hipGraphExec_t graph = nullptr;
-   hipGraph_t graph_def = nullptr;
-
-   double *kappa() { return scal; }
-   double *rr() { return scal + kMaxRhs; }
-   double *rr_new() { return scal + 2 * kMaxRhs; }
-   double *pap() { return scal + 3 * kMaxRhs; }
-   double *alpha() { return scal + 4 * kMaxRhs; }
-   double *beta() { return scal + 5 * kMaxRhs; }
-   double *tmp() { return scal + 6 * kMaxRhs; }
-   int *active() { return imask; }
-   int *update() { return imask + kMaxRhs; }
-
-   ~Workspace()
-   {
-      if (graph) (void)hipGraphExecDestroy(graph);
-      if (graph_def) (void)hipGraphDestroy(graph_def);
-      const bool alias = (t1 == qs);
-      for (plqcd_hip_field *f : {r, rs, ps, qs, aps, xs}) plqcd_hip_field_destroy(f);
-      if (!alias) {
-         plqcd_hip_field_destroy(t1);
-         plqcd_hip_field_destroy(t2);
-      }
-      if (partial) (void)hipFree(partial);
-      if (scal) (void)hipFree(scal);
-      if (imask) (void)hipFree(imask);
-      if (host) (void)hipHostFree(host);
-   }
-};
-
-int make_field(plqcd_hip_lattice *lat, int prec, int R, plqcd_hip_field **f)
+// rr[s] = rr_new[s] = src[s] for the slots in mask
+__global__ void k_admit(const int *mask, const double *src, double *rr, double *rr_new, int S)
 {
-   return plqcd_hip_field_create(lat, prec, R, f);
+   const int r = threadIdx.x;
+   if (r < S && mask[r]) rr[r] = rr_new[r] = src[r];
 }
 
-int alloc_workspace(Workspace &w, plqcd_hip_lattice *lat, int sloppy, int R)
+}  // namespace
+
+namespace plqcd {
+
+size_t CgEngine::bytes_needed(const plqcd_hip_lattice *lat, int sloppy, int S)
 {
-   w.lat = lat;
-   w.R = R;
-   int rc = make_field(lat, PLQCD_PREC_F64, R, &w.r);
-   if (!rc) rc = make_field(lat, sloppy, R, &w.rs);
-   if (!rc) rc = make_field(lat, sloppy, R, &w.ps);
-   if (!rc) rc = make_field(lat, sloppy, R, &w.qs);
-   if (!rc) rc = make_field(lat, sloppy, R, &w.aps);
-   if (!rc) rc = make_field(lat, sloppy, R, &w.xs);
-   if (rc) return rc;
+   const size_t f64 = 2 * spinor_parity_bytes(PLQCD_PREC_F64, lat->Vh) * S;
+   const size_t fs = 2 * spinor_parity_bytes(sloppy, lat->Vh) * S;
+   const size_t nf64 = sloppy == PLQCD_PREC_F64 ? 2 : 4;  // b, x (+ t1, t2)
+   return nf64 * f64 + 5 * fs + (size_t)S * 2 * reduce_chunks(lat->Vh) * sizeof(double) + (1 << 20);
+}
+
+int CgEngine::create(const plqcd_hip_gauge *g, const plqcd_hip_gauge *gs, int S, double delta, bool use_graph,
+                     plqcd_hip_field *b, plqcd_hip_field *x, CgEngine **out)
+{
+   *out = nullptr;
+   if (!g || !gs) return set_error(1, "cg: null gauge");
+   if (g->prec != PLQCD_PREC_F64) return set_error(1, "cg: the true-residual gauge field must be F64");
+   if (gs->lat != g->lat) return set_error(1, "cg: gauge fields on different lattices");
+   if (S < 1 || S > kMaxSlots) return set_error(2, "cg: 1 to %d right-hand sides per engine", kMaxSlots);
+   if ((b == nullptr) != (x == nullptr)) return set_error(1, "cg: borrow both b and x or neither");
+   if (b && (b->prec != PLQCD_PREC_F64 || x->prec != PLQCD_PREC_F64 || b->nrhs != S || x->nrhs != S ||
+             b->lat != g->lat || x->lat != g->lat))
+      return set_error(1, "cg: b and x must be F64 fields with the engine's RHS count on the gauge's lattice");
+   CgEngine *e = new (std::nothrow) CgEngine();
+   if (!e) return set_error(4, "out of host memory");
+   e->lat_ = g->lat;
+   e->g_ = g;
+   e->gs_ = gs;
+   e->S_ = S;
+   e->use_graph_ = use_graph;
+   const bool mixed = gs != g;
+   e->delta_ = delta > 0 ? delta : (!mixed ? 0.0 : (gs->prec == PLQCD_PREC_F16 ? 0.1 : 1e-3));
+   e->b_ = b;
+   e->x_ = x;
+   e->own_bx_ = b == nullptr;
+   e->slot_.assign(S, CgSlot{});
+   e->active_.assign(S, 0);
+   e->update_.assign(S, 0);
+   if (int rc = e->alloc(gs->prec)) {
+      delete e;
+      return rc;
+   }
+   *out = e;
+   return 0;
+}
+
+int CgEngine::alloc(int sloppy)
+{
+   if (own_bx_) {
+      if (int rc = plqcd_hip_field_create(lat_, PLQCD_PREC_F64, S_, &b_)) return rc;
+      if (int rc = plqcd_hip_field_create(lat_, PLQCD_PREC_F64, S_, &x_)) return rc;
+   }
+   for (plqcd_hip_field **f : {&rs_, &ps_, &qs_, &aps_, &xs_})
+      if (int rc = plqcd_hip_field_create(lat_, sloppy, S_, f)) return rc;
    if (sloppy == PLQCD_PREC_F64) {  // the sloppy temporaries are free during a reliable update
-      w.t1 = w.qs;
-      w.t2 = w.aps;
+      t1_ = qs_;
+      t2_ = aps_;
    } else {
-      rc = make_field(lat, PLQCD_PREC_F64, R, &w.t1);
-      if (!rc) rc = make_field(lat, PLQCD_PREC_F64, R, &w.t2);
-      if (rc) return rc;
+      if (int rc = plqcd_hip_field_create(lat_, PLQCD_PREC_F64, S_, &t1_)) return rc;
+      if (int rc = plqcd_hip_field_create(lat_, PLQCD_PREC_F64, S_, &t2_)) return rc;
    }
-   PLQCD_HIP_TRY(hipMalloc(&w.partial, plqcd::partial_doubles(w.r) * sizeof(double)));
-   PLQCD_HIP_TRY(hipMalloc(&w.scal, 7 * kMaxRhs * sizeof(double)));
-   PLQCD_HIP_TRY(hipMalloc(&w.imask, 2 * kMaxRhs * sizeof(int)));
-   PLQCD_HIP_TRY(hipHostMalloc(&w.host, 2 * kMaxRhs * sizeof(double), hipHostMallocDefault));
-   return 0;
-}
-
-// One CG iteration on the sloppy fields (see the file comment).
-int iteration(Workspace &w, const plqcd_hip_gauge *gs)
-{
-   hipStream_t st = w.lat->stream;
-   const int nc = plqcd::reduce_chunks(w.lat->Vh);
-   hipLaunchKernelGGL(k_beta, dim3(1), dim3(kMaxRhs), 0, st, w.rr(), w.rr_new(), w.active(), w.beta(), w.R);
-   if (int rc = plqcd::blas_xpay(w.rs, w.beta(), w.ps)) return rc;
-   plqcd::DslashOp m;
-   m.kappa = w.kappa();
-   m.partial = w.partial;
-   if (int rc = plqcd::dslash(gs, w.ps, w.qs, PLQCD_PARITY_ALL, PLQCD_HOP_WILSON, m)) return rc;
-   if (int rc = plqcd::reduce_partials(w.partial, 2 * nc, 0, 2 * nc, w.R, w.pap(), st)) return rc;
-   plqcd::DslashOp md;
-   md.kappa = w.kappa();
-   md.dag = 1;
-   if (int rc = plqcd::dslash(gs, w.qs, w.aps, PLQCD_PARITY_ALL, PLQCD_HOP_WILSON, md)) return rc;
-   hipLaunchKernelGGL(k_alpha, dim3(1), dim3(kMaxRhs), 0, st, w.rr(), w.pap(), w.active(), w.alpha(), w.R);
-   if (int rc = plqcd::blas_cg_update(w.xs, w.ps, w.rs, w.aps, w.alpha(), w.partial)) return rc;
-   if (int rc = plqcd::reduce_partials(w.partial, 2 * nc, 0, 2 * nc, w.R, w.rr_new(), st)) return rc;
-   PLQCD_HIP_TRY(hipMemcpyAsync(w.host, w.rr_new(), w.R * sizeof(double), hipMemcpyDeviceToHost, st));
-   PLQCD_HIP_TRY(hipGetLastError());
-   return 0;
-}
-
-// x += xs ; r = b - M^+M x ; rs = r ; tmp <- |r|^2, for the RHS in `update`.
-int reliable_update(Workspace &w, const plqcd_hip_gauge *g, const plqcd_hip_field *b, plqcd_hip_field *x)
-{
-   hipStream_t st = w.lat->stream;
-   const int nc = plqcd::reduce_chunks(w.lat->Vh);
-   if (int rc = plqcd::blas_accum(x, w.xs, 1, w.update())) return rc;
-   plqcd::DslashOp m;
-   m.kappa = w.kappa();
-   if (int rc = plqcd::dslash(g, x, w.t1, PLQCD_PARITY_ALL, PLQCD_HOP_WILSON, m)) return rc;
-   m.dag = 1;
-   if (int rc = plqcd::dslash(g, w.t1, w.t2, PLQCD_PARITY_ALL, PLQCD_HOP_WILSON, m)) return rc;
-   if (int rc = plqcd::blas_residual(b, w.t2, w.r, w.partial, w.update())) return rc;
-   if (int rc = plqcd::reduce_partials(w.partial, 2 * nc, 0, 2 * nc, w.R, w.tmp(), st)) return rc;
-   if (int rc = plqcd::blas_copy(w.rs, w.r, nullptr, w.update())) return rc;
-   PLQCD_HIP_TRY(hipMemcpyAsync(w.host + kMaxRhs, w.tmp(), w.R * sizeof(double), hipMemcpyDeviceToHost, st));
+   PLQCD_HIP_TRY(hipMalloc(&partial_, partial_doubles(b_) * sizeof(double)));
+   PLQCD_HIP_TRY(hipMalloc(&scal_, 7 * S_ * sizeof(double)));
+   PLQCD_HIP_TRY(hipMalloc(&imask_, 2 * S_ * sizeof(int)));
+   PLQCD_HIP_TRY(hipHostMalloc(&host_, 2 * S_ * sizeof(double), hipHostMallocDefault));
+   hipStream_t st = lat_->stream;
+   std::vector<double> z(7 * S_, 0.0);
+   PLQCD_HIP_TRY(hipMemcpyAsync(scal_, z.data(), z.size() * sizeof(double), hipMemcpyHostToDevice, st));
+   std::vector<int> zi(2 * S_, 0);
+   PLQCD_HIP_TRY(hipMemcpyAsync(imask_, zi.data(), zi.size() * sizeof(int), hipMemcpyHostToDevice, st));
+   if (int rc = blas_zero(ps_, nullptr)) return rc;
+   if (int rc = blas_zero(xs_, nullptr)) return rc;
+   if (int rc = blas_zero(rs_, nullptr)) return rc;
    PLQCD_HIP_TRY(hipStreamSynchronize(st));
    return 0;
 }
 
-}  // namespace
+CgEngine::~CgEngine()
+{
+   if (lat_) (void)hipStreamSynchronize(lat_->stream);
+   if (graph_) (void)hipGraphExecDestroy(graph_);
+   if (graph_def_) (void)hipGraphDestroy(graph_def_);
+   const bool alias = (t1_ == qs_);
+   for (plqcd_hip_field *f : {rs_, ps_, qs_, aps_, xs_}) plqcd_hip_field_destroy(f);
+   if (!alias) {
+      plqcd_hip_field_destroy(t1_);
+      plqcd_hip_field_destroy(t2_);
+   }
+   if (own_bx_) {
+      plqcd_hip_field_destroy(b_);
+      plqcd_hip_field_destroy(x_);
+   }
+   if (partial_) (void)hipFree(partial_);
+   if (scal_) (void)hipFree(scal_);
+   if (imask_) (void)hipFree(imask_);
+   if (host_) (void)hipHostFree(host_);
+}
+
+int CgEngine::free_slot() const
+{
+   for (int s = 0; s < S_; s++)
+      if (!slot_[s].active && !slot_[s].done) return s;
+   return -1;
+}
+
+int CgEngine::active_count() const
+{
+   int n = 0;
+   for (const CgSlot &s : slot_) n += s.active;
+   return n;
+}
+
+int CgEngine::admit(const std::vector<int> &which, const double *kappa, const double *tol, const int *max_iter)
+{
+   hipStream_t st = lat_->stream;
+   const int nc = reduce_chunks(lat_->Vh);
+   std::vector<int> mask(S_, 0);
+   for (size_t k = 0; k < which.size(); k++) {
+      const int s = which[k];
+      if (s < 0 || s >= S_ || slot_[s].active || slot_[s].done) return set_error(1, "cg: slot %d is not free", s);
+      mask[s] = 1;
+      CgSlot &sl = slot_[s];
+      sl = CgSlot{};
+      sl.kappa = kappa[k];
+      sl.tol = tol[k];
+      sl.max_iter = max_iter[k];
+   }
+   PLQCD_HIP_TRY(hipMemcpyAsync(update_d(), mask.data(), S_ * sizeof(int), hipMemcpyHostToDevice, st));
+   // x = xs = ps = 0 ; rs = b ; |b|^2 (in FP64, before rounding to the sloppy precision)
+   if (int rc = blas_zero(x_, update_d())) return rc;
+   if (int rc = blas_zero(xs_, update_d())) return rc;
+   if (int rc = blas_zero(ps_, update_d())) return rc;
+   if (int rc = blas_copy(rs_, b_, partial_, update_d())) return rc;
+   if (int rc = reduce_partials(partial_, 2 * nc, 0, 2 * nc, S_, tmp_d(), st)) return rc;
+   hipLaunchKernelGGL(k_admit, dim3(1), dim3(kMaxSlots), 0, st, update_d(), tmp_d(), rr_d(), rr_new_d(), S_);
+   PLQCD_HIP_TRY(hipGetLastError());
+   PLQCD_HIP_TRY(hipMemcpyAsync(host_ + S_, tmp_d(), S_ * sizeof(double), hipMemcpyDeviceToHost, st));
+   PLQCD_HIP_TRY(hipStreamSynchronize(st));
+   std::vector<double> kap(S_);
+   for (int s = 0; s < S_; s++) kap[s] = slot_[s].kappa;
+   for (int s : which) {
+      CgSlot &sl = slot_[s];
+      sl.bb = host_[S_ + s];
+      sl.rmax = std::sqrt(sl.bb);
+      if (sl.bb > 0.0) {
+         sl.active = true;
+      } else {  // b = 0: x = 0 is exact
+         sl.done = true;
+         sl.stats.converged = 1;
+      }
+      active_[s] = sl.active;
+   }
+   PLQCD_HIP_TRY(hipMemcpyAsync(kappa_d(), kap.data(), S_ * sizeof(double), hipMemcpyHostToDevice, st));
+   PLQCD_HIP_TRY(hipMemcpyAsync(active_d(), active_.data(), S_ * sizeof(int), hipMemcpyHostToDevice, st));
+   PLQCD_HIP_TRY(hipStreamSynchronize(st));
+   return 0;
+}
+
+int CgEngine::admit_host(int s, const spinor *b_host, double kappa, double tol, int max_iter)
+{
+   if (s < 0 || s >= S_) return set_error(1, "cg: bad slot %d", s);
+   if (int rc = field_upload_range(b_, s, b_host, 0, lat_->V)) return rc;
+   return admit(std::vector<int>{s}, &kappa, &tol, &max_iter);
+}
+
+// One CG iteration on the sloppy fields.
+int CgEngine::iteration()
+{
+   hipStream_t st = lat_->stream;
+   const int nc = reduce_chunks(lat_->Vh);
+   hipLaunchKernelGGL(k_beta, dim3(1), dim3(kMaxSlots), 0, st, rr_d(), rr_new_d(), active_d(), beta_d(), S_);
+   if (int rc = blas_xpay(rs_, beta_d(), ps_)) return rc;
+   DslashOp m;
+   m.kappa = kappa_d();
+   m.partial = partial_;
+   if (int rc = dslash(gs_, ps_, qs_, PLQCD_PARITY_ALL, PLQCD_HOP_WILSON, m)) return rc;
+   if (int rc = reduce_partials(partial_, 2 * nc, 0, 2 * nc, S_, pap_d(), st)) return rc;
+   DslashOp md;
+   md.kappa = kappa_d();
+   md.dag = 1;
+   if (int rc = dslash(gs_, qs_, aps_, PLQCD_PARITY_ALL, PLQCD_HOP_WILSON, md)) return rc;
+   hipLaunchKernelGGL(k_alpha, dim3(1), dim3(kMaxSlots), 0, st, rr_d(), pap_d(), active_d(), alpha_d(), S_);
+   if (int rc = blas_cg_update(xs_, ps_, rs_, aps_, alpha_d(), partial_)) return rc;
+   if (int rc = reduce_partials(partial_, 2 * nc, 0, 2 * nc, S_, rr_new_d(), st)) return rc;
+   PLQCD_HIP_TRY(hipMemcpyAsync(host_, rr_new_d(), S_ * sizeof(double), hipMemcpyDeviceToHost, st));
+   PLQCD_HIP_TRY(hipGetLastError());
+   return 0;
+}
+
+int CgEngine::capture()
+{
+   hipStream_t st = lat_->stream;
+   PLQCD_HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+   const int rc = iteration();
+   hipGraph_t gdef = nullptr;
+   const hipError_t e = hipStreamEndCapture(st, &gdef);
+   if (rc) {
+      if (gdef) (void)hipGraphDestroy(gdef);
+      return rc;
+   }
+   if (e != hipSuccess) return set_error(10, "cg: graph capture failed: %s", hipGetErrorString(e));
+   graph_def_ = gdef;
+   PLQCD_HIP_TRY(hipGraphInstantiate(&graph_, gdef, nullptr, nullptr, 0));
+   return 0;
+}
+
+int CgEngine::launch()
+{
+   if (use_graph_) {
+      if (!graph_)
+         if (int rc = capture()) return rc;
+      PLQCD_HIP_TRY(hipGraphLaunch(graph_, lat_->stream));
+      return 0;
+   }
+   return iteration();
+}
+
+// x += xs ; r = b - M^+M x (into t2) ; rs = r ; tmp <- |r|^2, for the slots in update.
+int CgEngine::reliable_update()
+{
+   hipStream_t st = lat_->stream;
+   const int nc = reduce_chunks(lat_->Vh);
+   if (int rc = blas_accum(x_, xs_, 1, update_d())) return rc;
+   DslashOp m;
+   m.kappa = kappa_d();
+   if (int rc = dslash(g_, x_, t1_, PLQCD_PARITY_ALL, PLQCD_HOP_WILSON, m)) return rc;
+   m.dag = 1;
+   if (int rc = dslash(g_, t1_, t2_, PLQCD_PARITY_ALL, PLQCD_HOP_WILSON, m)) return rc;
+   if (int rc = blas_residual(b_, t2_, t2_, partial_, update_d())) return rc;
+   if (int rc = reduce_partials(partial_, 2 * nc, 0, 2 * nc, S_, tmp_d(), st)) return rc;
+   if (int rc = blas_copy(rs_, t2_, nullptr, update_d())) return rc;
+   PLQCD_HIP_TRY(hipMemcpyAsync(host_ + S_, tmp_d(), S_ * sizeof(double), hipMemcpyDeviceToHost, st));
+   PLQCD_HIP_TRY(hipStreamSynchronize(st));
+   return 0;
+}
+
+int CgEngine::finish(std::vector<int> &finished)
+{
+   hipStream_t st = lat_->stream;
+   PLQCD_HIP_TRY(hipStreamSynchronize(st));
+   int nupd = 0;
+   for (int s = 0; s < S_; s++) {
+      update_[s] = 0;
+      CgSlot &sl = slot_[s];
+      if (!sl.active) continue;
+      sl.stats.iterations++;
+      const double res = std::sqrt(host_[s]);
+      if (res < delta_ * sl.rmax || res <= sl.tol * std::sqrt(sl.bb) || sl.stats.iterations >= sl.max_iter ||
+          !std::isfinite(res)) {
+         update_[s] = 1;
+         nupd++;
+      }
+   }
+   if (!nupd) return 0;
+   PLQCD_HIP_TRY(hipMemcpyAsync(update_d(), update_.data(), S_ * sizeof(int), hipMemcpyHostToDevice, st));
+   if (int rc = reliable_update()) return rc;
+   for (int s = 0; s < S_; s++) {
+      if (!update_[s]) continue;
+      CgSlot &sl = slot_[s];
+      const double rt = host_[S_ + s];
+      sl.stats.reliable_updates++;
+      sl.stats.true_residual = std::sqrt(rt / sl.bb);
+      sl.rmax = std::sqrt(rt);
+      host_[s] = rt;  // the true |r|^2 replaces the iterated one before the next beta
+      if (sl.stats.true_residual <= sl.tol) sl.stats.converged = 1;
+      if (sl.stats.converged || sl.stats.iterations >= sl.max_iter || !std::isfinite(rt)) {
+         sl.active = false;
+         sl.done = true;
+         active_[s] = 0;
+         finished.push_back(s);
+      }
+   }
+   PLQCD_HIP_TRY(hipMemcpyAsync(rr_new_d(), host_, S_ * sizeof(double), hipMemcpyHostToDevice, st));
+   PLQCD_HIP_TRY(hipMemcpyAsync(active_d(), active_.data(), S_ * sizeof(int), hipMemcpyHostToDevice, st));
+   PLQCD_HIP_TRY(hipStreamSynchronize(st));
+   return 0;
+}
+
+int CgEngine::retire(int s, spinor *x_host, plqcd_cg_stats *st)
+{
+   if (s < 0 || s >= S_ || !slot_[s].done) return set_error(1, "cg: slot %d has no result", s);
+   if (x_host)
+      if (int rc = field_download_range(x_, s, x_host, 0, lat_->V)) return rc;
+   CgSlot &sl = slot_[s];
+   sl.stats.hop_applications = 2.0 * (sl.stats.iterations + sl.stats.reliable_updates);
+   if (st) *st = sl.stats;
+   sl = CgSlot{};
+   return 0;
+}
+
+}  // namespace plqcd
 
 extern "C" int plqcd_hip_cg_solve(const plqcd_hip_gauge *g, const plqcd_hip_gauge *g_sloppy, const plqcd_hip_field *b,
                                   plqcd_hip_field *x, const plqcd_cg_params *prm, plqcd_cg_stats *stats)
@@ -169,120 +353,34 @@ extern "C" int plqcd_hip_cg_solve(const plqcd_hip_gauge *g, const plqcd_hip_gaug
    plqcd::clear_error();
    if (!g || !b || !x || !prm || !stats) return plqcd::set_error(1, "cg: null argument");
    const plqcd_hip_gauge *gs = g_sloppy ? g_sloppy : g;
-   plqcd_hip_lattice *lat = g->lat;
    const int R = b->nrhs;
-   if (g->prec != PLQCD_PREC_F64 || b->prec != PLQCD_PREC_F64 || x->prec != PLQCD_PREC_F64)
-      return plqcd::set_error(1, "cg: the gauge field, b and x must be F64");
-   if (gs->lat != lat || b->lat != lat || x->lat != lat) return plqcd::set_error(1, "cg: lattice mismatch");
    if (x->nrhs != R) return plqcd::set_error(1, "cg: b and x have different RHS counts");
-   if (R > kMaxRhs) return plqcd::set_error(2, "cg: at most %d right-hand sides per solve", kMaxRhs);
    if (prm->max_iter <= 0) return plqcd::set_error(2, "cg: max_iter must be positive");
-   const int sloppy = gs->prec;
-   const bool mixed = gs != g;
-   double delta = prm->delta;
-   if (delta <= 0.0) delta = !mixed ? 0.0 : (sloppy == PLQCD_PREC_F16 ? 0.1 : 1e-3);
-
-   Workspace w;
-   if (int rc = alloc_workspace(w, lat, sloppy, R)) return rc;
-   hipStream_t st = lat->stream;
-   const int nc = plqcd::reduce_chunks(lat->Vh);
-
-   std::vector<double> kap(kMaxRhs, 0.0), tol(R), bb(R), rmax(R), rr_new(R);
-   std::vector<int> active(kMaxRhs, 0), update(kMaxRhs, 0);
+   plqcd::CgEngine *e = nullptr;
+   if (int rc = plqcd::CgEngine::create(g, gs, R, prm->delta, prm->use_graph != 0,
+                                        const_cast<plqcd_hip_field *>(b), x, &e))
+      return rc;
+   std::vector<int> all(R);
+   std::vector<double> kap(R), tol(R);
+   std::vector<int> mi(R, prm->max_iter);
    for (int r = 0; r < R; r++) {
+      all[r] = r;
       kap[r] = prm->kappa_rhs ? prm->kappa_rhs[r] : prm->kappa;
       tol[r] = prm->tol_rhs ? prm->tol_rhs[r] : prm->tol;
-      stats[r] = plqcd_cg_stats{};
    }
-   PLQCD_HIP_TRY(hipMemcpyAsync(w.kappa(), kap.data(), kMaxRhs * sizeof(double), hipMemcpyHostToDevice, st));
-
-   // x = 0 ; r = b ; |b|^2 ; rs = r ; ps = 0 ; xs = 0 ; rr = rr_new = |b|^2
-   int rc = plqcd::blas_zero(x, nullptr);
-   if (!rc) rc = plqcd::blas_copy(w.r, b, w.partial, nullptr);
-   if (!rc) rc = plqcd::reduce_partials(w.partial, 2 * nc, 0, 2 * nc, R, w.rr(), st);
-   if (!rc) rc = plqcd::blas_copy(w.rs, w.r, nullptr, nullptr);
-   if (!rc) rc = plqcd::blas_zero(w.ps, nullptr);
-   if (!rc) rc = plqcd::blas_zero(w.xs, nullptr);
-   if (rc) return rc;
-   PLQCD_HIP_TRY(hipMemcpyAsync(w.rr_new(), w.rr(), R * sizeof(double), hipMemcpyDeviceToDevice, st));
-   PLQCD_HIP_TRY(hipMemcpyAsync(w.host, w.rr(), R * sizeof(double), hipMemcpyDeviceToHost, st));
-   PLQCD_HIP_TRY(hipStreamSynchronize(st));
-   int nactive = 0;
-   for (int r = 0; r < R; r++) {
-      bb[r] = w.host[r];
-      rmax[r] = std::sqrt(bb[r]);
-      active[r] = bb[r] > 0.0;
-      if (!active[r]) stats[r].converged = 1;  // b = 0: x = 0 is exact
-      nactive += active[r];
-   }
-   PLQCD_HIP_TRY(hipMemcpyAsync(w.active(), active.data(), kMaxRhs * sizeof(int), hipMemcpyHostToDevice, st));
-
-   // capture one iteration
-   if (prm->use_graph) {
-      PLQCD_HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-      rc = iteration(w, gs);
-      hipGraph_t gdef = nullptr;
-      hipError_t e = hipStreamEndCapture(st, &gdef);
-      if (rc) {
-         if (gdef) (void)hipGraphDestroy(gdef);
-         return rc;
-      }
-      if (e != hipSuccess) return plqcd::set_error(10, "cg: graph capture failed: %s", hipGetErrorString(e));
-      w.graph_def = gdef;
-      PLQCD_HIP_TRY(hipGraphInstantiate(&w.graph, gdef, nullptr, nullptr, 0));
-   }
-
-   int iter = 0;
-   while (nactive > 0 && iter < prm->max_iter) {
-      if (w.graph) PLQCD_HIP_TRY(hipGraphLaunch(w.graph, st));
-      else if ((rc = iteration(w, gs))) return rc;
-      PLQCD_HIP_TRY(hipStreamSynchronize(st));
-      iter++;
-      const bool last = iter == prm->max_iter;
-      int nupd = 0;
-      for (int r = 0; r < R; r++) {
-         update[r] = 0;
-         if (!active[r]) continue;
-         stats[r].iterations++;
-         rr_new[r] = w.host[r];
-         const double res = std::sqrt(rr_new[r]);
-         if (res < delta * rmax[r] || res <= tol[r] * std::sqrt(bb[r]) || last || !std::isfinite(res)) {
-            update[r] = 1;
-            nupd++;
-         }
-      }
-      if (nupd) {
-         PLQCD_HIP_TRY(hipMemcpyAsync(w.update(), update.data(), kMaxRhs * sizeof(int), hipMemcpyHostToDevice, st));
-         if ((rc = reliable_update(w, g, b, x))) return rc;
-         for (int r = 0; r < R; r++) {
-            if (!update[r]) continue;
-            const double rt = w.host[kMaxRhs + r];
-            stats[r].reliable_updates++;
-            stats[r].true_residual = std::sqrt(rt / bb[r]);
-            rr_new[r] = rt;
-            rmax[r] = std::sqrt(rt);
-            if (stats[r].true_residual <= tol[r]) {
-               stats[r].converged = 1;
-               active[r] = 0;
-               nactive--;
-            } else if (last || !std::isfinite(rt)) {
-               active[r] = 0;
-               nactive--;
-            }
-            w.host[r] = rr_new[r];
-         }
-         // the true |r|^2 replaces the iterated one before the next beta
-         PLQCD_HIP_TRY(hipMemcpyAsync(w.rr_new(), w.host, R * sizeof(double), hipMemcpyHostToDevice, st));
-         PLQCD_HIP_TRY(hipMemcpyAsync(w.active(), active.data(), kMaxRhs * sizeof(int), hipMemcpyHostToDevice, st));
-         PLQCD_HIP_TRY(hipStreamSynchronize(st));
-      }
+   int rc = e->admit(all, kap.data(), tol.data(), mi.data());
+   std::vector<int> finished;
+   while (!rc && e->active_count() > 0) {
+      rc = e->launch();
+      if (!rc) rc = e->finish(finished);
    }
    const double secs = std::chrono::duration<double>(clock::now() - t_start).count();
-   for (int r = 0; r < R; r++) {
+   for (int r = 0; r < R && !rc; r++) {
+      rc = e->retire(r, nullptr, &stats[r]);
       stats[r].seconds = secs;
-      stats[r].hop_applications = 2.0 * (stats[r].iterations + stats[r].reliable_updates);
    }
-   return 0;
+   delete e;
+   return rc;
 }
 
 extern "C" int plqcd_hip_apply_m(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip_field *out,
@@ -290,7 +388,6 @@ extern "C" int plqcd_hip_apply_m(const plqcd_hip_gauge *g, const plqcd_hip_field
 {
    if (!g || !in || !out || !kappa_rhs) return plqcd::set_error(1, "apply_m: null argument");
    const int R = in->nrhs;
-   if (R > kMaxRhs) return plqcd::set_error(2, "apply_m: at most %d right-hand sides", kMaxRhs);
    plqcd_hip_lattice *lat = g->lat;
    double *dk = nullptr;
    PLQCD_HIP_TRY(hipMallocAsync((void **)&dk, R * sizeof(double), lat->stream));
@@ -300,6 +397,31 @@ extern "C" int plqcd_hip_apply_m(const plqcd_hip_gauge *g, const plqcd_hip_field
    op.kappa = dk;
    op.dag = dagger ? 1 : 0;
    if (!rc) rc = plqcd::dslash(g, in, out, PLQCD_PARITY_ALL, PLQCD_HOP_WILSON, op);
+   (void)hipFreeAsync(dk, lat->stream);
+   if (!rc) PLQCD_HIP_TRY(hipStreamSynchronize(lat->stream));
+   return rc;
+}
+
+extern "C" int plqcd_hip_apply_schur(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip_field *tmp,
+                                     plqcd_hip_field *out, const double *kappa_rhs, int parity)
+{
+   if (!g || !in || !tmp || !out || !kappa_rhs) return plqcd::set_error(1, "apply_schur: null argument");
+   if (parity != PLQCD_PARITY_EVEN && parity != PLQCD_PARITY_ODD)
+      return plqcd::set_error(1, "apply_schur: parity must be EVEN or ODD");
+   if (in == tmp || in == out || tmp == out) return plqcd::set_error(1, "apply_schur: fields must be distinct");
+   const int R = in->nrhs;
+   plqcd_hip_lattice *lat = g->lat;
+   std::vector<double> k2(R);
+   for (int r = 0; r < R; r++) k2[r] = kappa_rhs[r] * kappa_rhs[r];
+   double *dk = nullptr;
+   PLQCD_HIP_TRY(hipMallocAsync((void **)&dk, R * sizeof(double), lat->stream));
+   hipError_t e = hipMemcpyAsync(dk, k2.data(), R * sizeof(double), hipMemcpyHostToDevice, lat->stream);
+   int rc = e == hipSuccess ? 0 : plqcd::set_error(10, "apply_schur: %s", hipGetErrorString(e));
+   const int other = parity == PLQCD_PARITY_EVEN ? PLQCD_PARITY_ODD : PLQCD_PARITY_EVEN;
+   // tmp(other) = H in(parity) ; out(parity) = H tmp(other) ; out = in - kappa^2 out on parity
+   if (!rc) rc = plqcd_hip_hop(g, in, tmp, other, PLQCD_HOP_WILSON);
+   if (!rc) rc = plqcd_hip_hop(g, tmp, out, parity, PLQCD_HOP_WILSON);
+   if (!rc) rc = plqcd::blas_xmcy_parity(in, out, dk, parity);
    (void)hipFreeAsync(dk, lat->stream);
    if (!rc) PLQCD_HIP_TRY(hipStreamSynchronize(lat->stream));
    return rc;

@@ -17,6 +17,13 @@ def apply_m(gauge: Gauge, fin: Field, fout: Field, kappa, dagger: bool = False) 
           "apply_m")
 
 
+def apply_schur(gauge: Gauge, fin: Field, ftmp: Field, fout: Field, kappa, parity: int = 0) -> None:
+    """fout_p = fin_p - kappa^2 H_pq H_qp fin_p on parity p (0 even, 1 odd)."""
+    k = np.broadcast_to(np.asarray(kappa, dtype=np.float64), (fin.nrhs,)).copy()
+    check(lib().plqcd_hip_apply_schur(gauge.h, fin.h, ftmp.h, fout.h, k.ctypes.data_as(C.POINTER(C.c_double)),
+                                      int(parity)), "apply_schur")
+
+
 def cg_solve(gauge: Gauge, b: Field, x: Field, kappa, tol, max_iter: int = 20000, sloppy: Gauge | None = None,
              delta: float = 0.0, use_graph: bool = True) -> list[dict]:
     """Solve M^+M x = b for every RHS of b.  kappa and tol may be scalars or

@@ -118,3 +118,29 @@ def test_cg_benchmark_shape_32c64(hp):
     print(st)
     assert st["converged"]
     assert osv.true_residual(bs[0], x.download(), u, L, KAPPA) <= 1.05e-10
+
+
+@pytest.mark.parametrize("parity", [0, 1])
+def test_schur_operator_matches_oracle_composition(hp, parity):
+    """1 - kappa^2 H_pq H_qp from the parity-checked halves (oracle eo/oe)."""
+    from plqcd_amd import solver as sv
+    from oracle import MODE_WILSON, hopping_eo, hopping_oe
+    L = (4, 6, 4, 8)
+    V = int(np.prod(L))
+    lat, g, b, u, bs = setup(hp, L, nrhs=2)
+    tmp, out = hp.Field(lat, nrhs=2), hp.Field(lat, nrhs=2)
+    sentinel = hp.random_spinor(V, 5)
+    for r in range(2):
+        out.upload(sentinel, r)
+    kaps = [0.12, 0.124]
+    sv.apply_schur(g, b, tmp, out, kaps, parity)
+    half = slice(0, V // 2) if parity == 0 else slice(V // 2, V)
+    other = slice(V // 2, V) if parity == 0 else slice(0, V // 2)
+    first, second = (hopping_oe, hopping_eo) if parity == 0 else (hopping_eo, hopping_oe)
+    for r in range(2):
+        q = bs[r]
+        t = first(q, u, L, MODE_WILSON)
+        ref = q - kaps[r] ** 2 * second(t, u, L, MODE_WILSON)
+        got = out.download(r)
+        assert site_rel_err(got[half], ref[half]) < 1e-13
+        assert np.array_equal(got[other], sentinel[other])
