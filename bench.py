@@ -246,13 +246,14 @@ def solver_suite():
         for r in range(R):
             fi.upload(hp.random_spinor(V, 10 + r), r)
         t_def = time_hop(hp, g, fi, fo)
-        best = (C.c_int * 4)()
+        best = (C.c_int * 5)()
         bms = C.c_double()
         capi.check(lib.plqcd_hip_autotune(g.h, fi.h, fo.h, 5, best, C.byref(bms)), "autotune")
         t_tuned = time_hop(hp, g, fi, fo)
         emit({"suite": "solver", "case": f"autotune_32c64_{PREC_NAMES[prec]}_l{links}_R{R}",
               "default_us": round(t_def * 1e3, 1), "tuned_us": round(t_tuned * 1e3, 1),
-              "best": {"order": best[0], "group": best[1], "block": best[2], "tmarch": best[3]}})
+              "best": {"order": best[0], "group": best[1], "block": best[2], "tmarch": best[3],
+                       "rb": best[4]}})
         fi.close(); fo.close(); g.close()
     lib.plqcd_hip_autotune_clear()
     lat.close()

@@ -301,19 +301,22 @@ void plqcd_hip_server_destroy(plqcd_hip_server *s);
  *   consecutive tiles; block: threads per block (64, 128, 256 or 512);
  *   tmarch > 0: full-lattice wilson applications whose time extent is a
  *   multiple of tmarch use the t-march kernel (one thread walks tmarch
- *   t-slices of a column), otherwise the one-site-per-thread gather kernel.
- * Results do not depend on it beyond rounding.  Returns 0, or nonzero for
- * bad values. */
-int plqcd_hip_hop_tuning(int prec, int order, int group, int block, int tmarch);
-/* Current launch shape of precision `prec`: out = {order, group, block, tmarch}. */
-int plqcd_hip_hop_get_tuning(int prec, int out[4]);
+ *   t-slices of a column), otherwise the one-site-per-thread gather kernel;
+ *   rb (1, 2 or 4): right-hand sides per thread of the gather kernel when it
+ *   divides the field's RHS count (links are loaded once per rb RHS; the
+ *   t-march kernel is used only with rb 1).
+ * Each RHS's result is bitwise the same for every launch shape.  Returns 0,
+ * or nonzero for bad values. */
+int plqcd_hip_hop_tuning(int prec, int order, int group, int block, int tmarch, int rb);
+/* Current launch shape of precision `prec`: out = {order, group, block, tmarch, rb}. */
+int plqcd_hip_hop_get_tuning(int prec, int out[5]);
 /* Times every launch shape for the shape of (g, in) -- lattice, precision,
  * link format, RHS count -- with `reps` timed applications each, and makes
  * the fastest the launch shape of that shape from now on.  best = {order,
- * group, block, tmarch}, *best_ms = its time per application.  in/out are
- * overwritten. */
+ * group, block, tmarch, rb}, *best_ms = its time per application.  in/out
+ * are overwritten. */
 int plqcd_hip_autotune(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip_field *out, int reps,
-                       int best[4], double *best_ms);
+                       int best[5], double *best_ms);
 /* Forgets every autotuned shape (back to the per-precision defaults). */
 int plqcd_hip_autotune_clear(void);
 

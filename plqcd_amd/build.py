@@ -28,7 +28,7 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wextra", "-Wno-unused-paramet
 # Packed FP32 instructions are kept out of all device code.
 HIP_FLAGS = ["-x", "hip", f"--offload-arch={ARCH}", "-fvisibility=hidden",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-             "-munsafe-fp-atomics", "-ffp-contract=fast"]
+             "-munsafe-fp-atomics", "-ffp-contract=off"]
 HOST_FLAGS = ["-fvisibility=hidden"]
 
 

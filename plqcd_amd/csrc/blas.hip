@@ -56,7 +56,7 @@ __device__ __forceinline__ double norm2(const d2 s[12])
 {
    double n = 0.0;
 #pragma unroll
-   for (int c = 0; c < 12; c++) n += s[c].x * s[c].x + s[c].y * s[c].y;
+   for (int c = 0; c < 12; c++) n = fma(s[c].y, s[c].y, fma(s[c].x, s[c].x, n));
    return n;
 }
 
@@ -77,8 +77,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(SpinorIO<P> x, SpinorIO<P>
       load_d(A, p, j, as);
 #pragma unroll
       for (int c = 0; c < 12; c++) {
-         xs[c] += al * ps[c];
-         rs[c] -= al * as[c];
+         xs[c] = d2{fma(al, ps[c].x, xs[c].x), fma(al, ps[c].y, xs[c].y)};
+         rs[c] = d2{fma(-al, as[c].x, rs[c].x), fma(-al, as[c].y, rs[c].y)};
       }
       store_d(X, p, j, xs);
       store_d(Rr, p, j, rs);
@@ -99,7 +99,7 @@ __global__ __launch_bounds__(kBlock) void k_xpay(SpinorIO<P> rv, const double *b
    load_d(Rr, p, j, rs);
    load_d(Pp, p, j, ps);
 #pragma unroll
-   for (int c = 0; c < 12; c++) ps[c] = rs[c] + be * ps[c];
+   for (int c = 0; c < 12; c++) ps[c] = d2{fma(be, ps[c].x, rs[c].x), fma(be, ps[c].y, rs[c].y)};
    store_d(Pp, p, j, ps);
 }
 
@@ -187,7 +187,7 @@ __global__ __launch_bounds__(kBlock) void k_xmcy_parity(SpinorIO<P> xv, SpinorIO
    load_d(X, p, j, xs);
    load_d(Y, p, j, ys);
 #pragma unroll
-   for (int m = 0; m < 12; m++) ys[m] = xs[m] - cc * ys[m];
+   for (int m = 0; m < 12; m++) ys[m] = d2{fma(-cc, ys[m].x, xs[m].x), fma(-cc, ys[m].y, xs[m].y)};
    store_d(Y, p, j, ys);
 }
 

@@ -124,12 +124,24 @@ __device__ __forceinline__ C cjmadd(C a, C b, C c)
    r.y = fma(-a.y, b.x, r.y);
    return r;
 }
+// All multiply-adds are explicit fma (the HIP code is built with
+// -ffp-contract=off), so a given site and RHS is computed by the same
+// operation sequence in every kernel variant.
 template <typename C>
 __device__ __forceinline__ C cmul(C a, C b)
 {
    C r;
-   r.x = a.x * b.x - a.y * b.y;
-   r.y = a.x * b.y + a.y * b.x;
+   r.x = fma(a.x, b.x, -(a.y * b.y));
+   r.y = fma(a.x, b.y, a.y * b.x);
+   return r;
+}
+// c - k*a, componentwise
+template <typename C, typename R>
+__device__ __forceinline__ C cxmky(C c, R k, C a)
+{
+   C r;
+   r.x = fma(-k, a.x, c.x);
+   r.y = fma(-k, a.y, c.y);
    return r;
 }
 // i*a

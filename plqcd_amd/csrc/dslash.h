@@ -25,6 +25,7 @@ struct HopArgs {
 struct HopTuning {
    int order, group, block;
    int tmarch;  // t-slices per thread of the t-march kernel (0: gather kernel)
+   int rb;      // right-hand sides per lane (1, 2 or 4; RHS-blocked gather when > 1)
 };
 extern HopTuning g_tuning[3];  // per storage precision
 
@@ -44,6 +45,14 @@ struct DslashOp {
 
 int dslash(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip_field *out, int parity, int mode,
            const DslashOp &op);
+
+// Launchers of one (storage precision, link format); arguments already checked.
+template <int PREC, int LINKS>
+int hop_launch(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip_field *out, int parity, int mode,
+               const DslashOp &op);
+template <int PREC, int LINKS>
+int hop_batch_launch(int n, const plqcd_hip_gauge *const *g, const plqcd_hip_field *const *in,
+                     plqcd_hip_field *const *out, int mode);
 
 // Launch shapes found by the autotuner, per (lattice, precision, links, RHS).
 struct TuneKey {

@@ -1,0 +1,654 @@
+// The Wilson hopping matrix on gfx950.
+//
+//   out(x) = sum_mu [ U_mu(x) (1 - gamma_mu) in(x+mu) + U_mu(x-mu)^+ (1 + gamma_mu) in(x-mu) ]
+//
+// (plqcd.pdf Eq. 2.1; lib/dirac.c:35-408 and :414-791 compute the two parity
+// halves of it in four phases over half-spinor scratch fields).  Here it is a
+// single gather pass: one thread per output site reads its 8 neighbour
+// spinors and 8 links straight from HBM-resident structure-of-arrays fields,
+// projects each neighbour to a half-spinor (2 colour vectors) before the SU(3)
+// multiply, and writes its output once with a non-temporal store.  There is
+// no scratch field, no halo and no MPI: periodic neighbours are computed from
+// the site coordinates inside the kernel.
+//
+// Gamma matrices, projectors and reconstruction are the reference's chiral
+// basis (lib/dirac.c:95-114, :163-195, :269-311, :371-398):
+//   mu  upper((1-g)f)          upper((1+g)g)           s2 +=            s3 +=
+//   0   (f0+f2,   f1+f3)       (g0-g2,   g1-g3)        a0 - b0          a1 - b1
+//   1   (f0+i f3, f1+i f2)     (g0-i g3, g1-i g2)      -i a1 + i b1     -i a0 + i b0
+//   2   (f0+f3,   f1-f2)       (g0-g3,   g1+g2)        -a1 + b1         a0 - b0
+//   3   (f0+i f2, f1-i f3)     (g0-i g2, g1+i g3)      -i a0 + i b0     i a1 - i b1
+// with a = U_mu(x) h, b = U_mu(x-mu)^+ k, s0 += a0 + b0, s1 += a1 + b1.
+//
+// MODE_COMPAT reproduces what the compiled reference outputs: forward terms
+// do not reach s2/s3 (dirac.c:373-374 and :756-757 assign the mu=0 backward
+// term over them) and odd output sites take their forward links from the even
+// site with the same index inside its parity (dirac.c:643-644).
+#pragma once
+
+#include <algorithm>
+#include <vector>
+
+#include "fields.h"
+#include "dslash.h"
+#include "reduce.h"
+
+namespace {
+
+constexpr int MODE_COMPAT = PLQCD_HOP_COMPAT;
+
+// Site of thread (parity p, index j inside the parity) and the within-parity
+// indices of its 8 neighbours (which all have the other parity).
+struct Nbr {
+   int up[4], dn[4];
+};
+
+__device__ __forceinline__ Nbr neighbours(const HopArgs &a, int p, int j)
+{
+   const int row = j / a.Lxh;  // row = y + L1*(z + L2*t)
+   const int xh = j - row * a.Lxh;
+   const int y = row % a.L[1];
+   const int zt = row / a.L[1];
+   const int z = zt % a.L[2];
+   const int t = zt / a.L[2];
+   const int x = 2 * xh + ((p + y + z + t) & 1);
+   Nbr n;
+   const int xp = (x + 1 == a.L[0]) ? 0 : x + 1;
+   const int xm = (x == 0) ? a.L[0] - 1 : x - 1;
+   n.up[0] = row * a.Lxh + (xp >> 1);
+   n.dn[0] = row * a.Lxh + (xm >> 1);
+   const int ryp = (y == a.L[1] - 1) ? row - (a.L[1] - 1) : row + 1;
+   const int rym = (y == 0) ? row + (a.L[1] - 1) : row - 1;
+   const int sz = a.L[1];
+   const int rzp = (z == a.L[2] - 1) ? row - (a.L[2] - 1) * sz : row + sz;
+   const int rzm = (z == 0) ? row + (a.L[2] - 1) * sz : row - sz;
+   const int st = a.L[1] * a.L[2];
+   const int rtp = (t == a.L[3] - 1) ? row - (a.L[3] - 1) * st : row + st;
+   const int rtm = (t == 0) ? row + (a.L[3] - 1) * st : row - st;
+   n.up[1] = ryp * a.Lxh + xh;
+   n.dn[1] = rym * a.Lxh + xh;
+   n.up[2] = rzp * a.Lxh + xh;
+   n.dn[2] = rzm * a.Lxh + xh;
+   n.up[3] = rtp * a.Lxh + xh;
+   n.dn[3] = rtm * a.Lxh + xh;
+   return n;
+}
+
+// h = upper((1 - gamma_mu) f): h[0..2] = first colour vector, h[3..5] = second
+template <int MU, class C>
+__device__ __forceinline__ void proj_minus(const C f[12], C h[6])
+{
+#pragma unroll
+   for (int c = 0; c < 3; c++) {
+      if (MU == 0) { h[c] = f[c] + f[6 + c];     h[3 + c] = f[3 + c] + f[9 + c]; }
+      if (MU == 1) { h[c] = f[c] + ci(f[9 + c]); h[3 + c] = f[3 + c] + ci(f[6 + c]); }
+      if (MU == 2) { h[c] = f[c] + f[9 + c];     h[3 + c] = f[3 + c] - f[6 + c]; }
+      if (MU == 3) { h[c] = f[c] + ci(f[6 + c]); h[3 + c] = f[3 + c] - ci(f[9 + c]); }
+   }
+}
+// k = upper((1 + gamma_mu) g)
+template <int MU, class C>
+__device__ __forceinline__ void proj_plus(const C g[12], C k[6])
+{
+#pragma unroll
+   for (int c = 0; c < 3; c++) {
+      if (MU == 0) { k[c] = g[c] - g[6 + c];     k[3 + c] = g[3 + c] - g[9 + c]; }
+      if (MU == 1) { k[c] = g[c] - ci(g[9 + c]); k[3 + c] = g[3 + c] - ci(g[6 + c]); }
+      if (MU == 2) { k[c] = g[c] - g[9 + c];     k[3 + c] = g[3 + c] + g[6 + c]; }
+      if (MU == 3) { k[c] = g[c] - ci(g[6 + c]); k[3 + c] = g[3 + c] + ci(g[9 + c]); }
+   }
+}
+// acc += forward reconstruction of a (lower components only if LOWER)
+template <int MU, bool LOWER, class C>
+__device__ __forceinline__ void recon_minus(C acc[12], const C a[6])
+{
+#pragma unroll
+   for (int c = 0; c < 3; c++) {
+      acc[c] += a[c];
+      acc[3 + c] += a[3 + c];
+      if (LOWER) {
+         if (MU == 0) { acc[6 + c] += a[c];          acc[9 + c] += a[3 + c]; }
+         if (MU == 1) { acc[6 + c] -= ci(a[3 + c]); acc[9 + c] -= ci(a[c]); }
+         if (MU == 2) { acc[6 + c] -= a[3 + c];     acc[9 + c] += a[c]; }
+         if (MU == 3) { acc[6 + c] -= ci(a[c]);     acc[9 + c] += ci(a[3 + c]); }
+      }
+   }
+}
+template <int MU, class C>
+__device__ __forceinline__ void recon_plus(C acc[12], const C b[6])
+{
+#pragma unroll
+   for (int c = 0; c < 3; c++) {
+      acc[c] += b[c];
+      acc[3 + c] += b[3 + c];
+      if (MU == 0) { acc[6 + c] -= b[c];          acc[9 + c] -= b[3 + c]; }
+      if (MU == 1) { acc[6 + c] += ci(b[3 + c]); acc[9 + c] += ci(b[c]); }
+      if (MU == 2) { acc[6 + c] += b[3 + c];     acc[9 + c] -= b[c]; }
+      if (MU == 3) { acc[6 + c] += ci(b[c]);     acc[9 + c] -= ci(b[3 + c]); }
+   }
+}
+
+// r = U v for the two colour vectors of a half-spinor
+template <class C>
+__device__ __forceinline__ void mul_u(const C u[9], const C v[6], C r[6])
+{
+#pragma unroll
+   for (int s = 0; s < 2; s++)
+#pragma unroll
+      for (int a = 0; a < 3; a++) {
+         C t = cmul(u[3 * a], v[3 * s]);
+         t = cmadd(u[3 * a + 1], v[3 * s + 1], t);
+         r[3 * s + a] = cmadd(u[3 * a + 2], v[3 * s + 2], t);
+      }
+}
+// r = U^+ v
+template <class C>
+__device__ __forceinline__ void mul_udag(const C u[9], const C v[6], C r[6])
+{
+#pragma unroll
+   for (int s = 0; s < 2; s++)
+#pragma unroll
+      for (int a = 0; a < 3; a++) {
+         C t = cmul(cconj(u[a]), v[3 * s]);
+         t = cjmadd(u[3 + a], v[3 * s + 1], t);
+         r[3 * s + a] = cjmadd(u[6 + a], v[3 * s + 2], t);
+      }
+}
+
+// (1 + SIGN gamma_mu) projection and reconstruction, SIGN = -1 or +1
+template <int MU, int SIGN, class C>
+__device__ __forceinline__ void proj(const C f[12], C h[6])
+{
+   if (SIGN < 0) proj_minus<MU>(f, h);
+   else proj_plus<MU>(f, h);
+}
+template <int MU, int SIGN, bool LOWER, class C>
+__device__ __forceinline__ void recon(C acc[12], const C a[6])
+{
+   if (SIGN < 0) recon_minus<MU, LOWER>(acc, a);
+   else recon_plus<MU>(acc, a);
+}
+
+// Forward term of direction MU for output site (p, j); jnb = site of x+mu.
+// H uses (1 - gamma_mu) forward and (1 + gamma_mu) backward; the adjoint
+// H^+ = gamma5 H gamma5 has the signs swapped (gamma5 anticommutes with
+// gamma_mu), so DAG only flips the projector signs.
+template <int MU, int MODE, bool DAG, class SIO, class GIO>
+__device__ __forceinline__ void hop_fwd(const SIO &in, const GIO &g, int p, int j, int jnb, typename SIO::C acc[12])
+{
+   typedef typename SIO::C C;
+   constexpr int SG = DAG ? 1 : -1;
+   C f[12], h[6], u[9], r[6];
+   in.load(1 - p, jnb, f);
+   proj<MU, SG>(f, h);
+   g.load(MODE == MODE_COMPAT ? 0 : p, MU, j, u);
+   mul_u(u, h, r);
+   recon<MU, SG, MODE != MODE_COMPAT>(acc, r);
+}
+
+// Backward term of direction MU; jnb = site of x-mu (its link U_mu(x-mu) too).
+template <int MU, bool DAG, class SIO, class GIO>
+__device__ __forceinline__ void hop_bwd(const SIO &in, const GIO &g, int p, int jnb, typename SIO::C acc[12])
+{
+   typedef typename SIO::C C;
+   constexpr int SG = DAG ? -1 : 1;
+   C f[12], k[6], u[9], r[6];
+   in.load(1 - p, jnb, f);
+   proj<MU, SG>(f, k);
+   g.load(1 - p, MU, jnb, u);
+   mul_udag(u, k, r);
+   recon<MU, SG, true>(acc, r);
+}
+
+template <int MU, int MODE, bool DAG, class SIO, class GIO>
+__device__ __forceinline__ void hop_dir(const SIO &in, const GIO &g, int p, int j, const Nbr &n,
+                                        typename SIO::C acc[12])
+{
+   hop_fwd<MU, MODE, DAG>(in, g, p, j, n.up[MU], acc);
+   hop_bwd<MU, DAG>(in, g, p, n.dn[MU], acc);
+}
+
+// Maps the hardware block index to a logical tile.  Blocks are handed to
+// the 8 XCDs round-robin (block b runs on XCD b % 8), and each XCD has its
+// own L2, so the order decides which L2 sees which part of the lattice:
+//   order 0: logical = b
+//   order 1: XCD k takes the k-th contiguous eighth of the lattice
+//   order 2: the lattice is cut into chunks of 8*group logical blocks; XCD k
+//            takes the k-th run of `group` blocks of every chunk, so all XCDs
+//            sweep the lattice together and neighbouring tiles share an L2
+// The output parity alternates between consecutive logical blocks, so a link
+// is read as a forward and as a backward link by nearby tiles.
+__device__ __forceinline__ int logical_block(const HopArgs &a)
+{
+   const int b = blockIdx.x, nb = gridDim.x;
+   if (a.order == 1) return (b & 7) * (nb >> 3) + (b >> 3);
+   if (a.order == 2) {
+      const int q = b >> 3, G = a.group;
+      return (q / G) * 8 * G + (b & 7) * G + (q % G);
+   }
+   return b;
+}
+
+// Epilogue: EPI = 0 stores H in; EPI = 1 stores in(x) - kappa_r * (H in)(x).
+// With NORM the site's |out|^2 joins the deterministic chunk reduction.
+template <int EPI, bool NORM, class SIO>
+__device__ __forceinline__ double epilogue(const SIO &in, const SIO &out, const HopArgs &a, int r, int p, int j,
+                                           typename SIO::C acc[12], const typename SIO::C *cen)
+{
+   typedef typename SIO::real real;
+   typedef typename SIO::C C;
+   if (EPI == 1) {
+      C c0[12];
+      if (cen == nullptr) in.load(p, j, c0);
+      const real kap = (real)a.kappa[r];
+#pragma unroll
+      for (int m = 0; m < 12; m++) acc[m] = cxmky(cen ? cen[m] : c0[m], kap, acc[m]);
+   }
+   double nrm = 0.0;
+   if (NORM) {
+#pragma unroll
+      for (int m = 0; m < 12; m++) nrm = fma((double)acc[m].y, (double)acc[m].y, fma((double)acc[m].x, (double)acc[m].x, nrm));
+   }
+   out.store(p, j, acc);
+   return nrm;
+}
+
+// One block of the gather kernel: logical block `blk` of the launch.
+template <int MODE, bool DAG, int EPI, bool NORM, int BLOCK, class SIO, class GIO>
+__device__ __forceinline__ void hop_block(const SIO &in, const SIO &out, const GIO &g, const HopArgs &a, int blk)
+{
+   typedef typename SIO::C C;
+   int p;
+   if (a.nparity == 2) {
+      p = blk & 1;
+      blk >>= 1;
+   } else {
+      p = a.parity0;
+   }
+   const int r = threadIdx.x % a.R;
+   const int js = threadIdx.x / a.R;
+   const int j = blk * a.spb + js;
+   double nrm = 0.0;
+   if (js < a.spb && j < a.Vh) {
+      const SIO si = in.at_rhs(r), so = out.at_rhs(r);
+      const Nbr n = neighbours(a, p, j);
+      C acc[12];
+#pragma unroll
+      for (int c = 0; c < 12; c++) acc[c] = C{0, 0};
+      hop_dir<0, MODE, DAG>(si, g, p, j, n, acc);
+      hop_dir<1, MODE, DAG>(si, g, p, j, n, acc);
+      hop_dir<2, MODE, DAG>(si, g, p, j, n, acc);
+      hop_dir<3, MODE, DAG>(si, g, p, j, n, acc);
+      nrm = epilogue<EPI, NORM>(si, so, a, r, p, j, acc, (const C *)nullptr);
+   }
+   if (NORM) chunk_reduce<BLOCK>(nrm, js, r, a.R, p * a.nchunk + (blk * a.spb + js) / REDUCE_CHUNK,
+                                 j - (js & (REDUCE_CHUNK - 1)) < a.Vh, a.partial + (size_t)r * 2 * a.nchunk);
+}
+
+template <int MODE, bool DAG, int EPI, bool NORM, int BLOCK, class SIO, class GIO>
+__global__ __launch_bounds__(BLOCK) void hop_kernel(SIO in, SIO out, GIO g, HopArgs a)
+{
+   hop_block<MODE, DAG, EPI, NORM, BLOCK>(in, out, g, a, logical_block(a));
+}
+
+// RHS-blocked gather: each lane handles RB consecutive right-hand sides of
+// one site, so a link is loaded (and, with 12-real links, rebuilt) once per
+// RB right-hand sides.  Lanes per site G = R / RB.  Every RHS goes through
+// the same operation sequence as in hop_block.
+template <int MODE, bool DAG, int EPI, bool NORM, int BLOCK, int RB, class SIO, class GIO>
+__device__ __forceinline__ void hop_block_rb(const SIO &in, const SIO &out, const GIO &g, const HopArgs &a, int blk)
+{
+   typedef typename SIO::C C;
+   constexpr int SF = DAG ? 1 : -1, SB = -SF;
+   int p;
+   if (a.nparity == 2) {
+      p = blk & 1;
+      blk >>= 1;
+   } else {
+      p = a.parity0;
+   }
+   const int G = a.R / RB;
+   const int gi = threadIdx.x % G;
+   const int js = threadIdx.x / G;
+   const int j = blk * a.spb + js;
+   double nrm[RB];
+#pragma unroll
+   for (int k = 0; k < RB; k++) nrm[k] = 0.0;
+   if (js < a.spb && j < a.Vh) {
+      const Nbr n = neighbours(a, p, j);
+      C acc[RB][12];
+#pragma unroll
+      for (int k = 0; k < RB; k++)
+#pragma unroll
+         for (int c = 0; c < 12; c++) acc[k][c] = C{0, 0};
+      const SIO si = in.at_rhs(gi * RB);
+#define PLQCD_RB_DIR(MU)                                                                      \
+   {                                                                                          \
+      C u[9];                                                                                 \
+      g.load(MODE == MODE_COMPAT ? 0 : p, MU, j, u);                                          \
+      _Pragma("unroll") for (int k = 0; k < RB; k++)                                          \
+      {                                                                                       \
+         C f[12], h[6], r[6];                                                                 \
+         si.at_rhs(gi * RB + k).load(1 - p, n.up[MU], f);                                     \
+         proj<MU, SF>(f, h);                                                                  \
+         mul_u(u, h, r);                                                                      \
+         recon<MU, SF, MODE != MODE_COMPAT>(acc[k], r);                                       \
+      }                                                                                       \
+      g.load(1 - p, MU, n.dn[MU], u);                                                         \
+      _Pragma("unroll") for (int k = 0; k < RB; k++)                                          \
+      {                                                                                       \
+         C f[12], h[6], r[6];                                                                 \
+         si.at_rhs(gi * RB + k).load(1 - p, n.dn[MU], f);                                     \
+         proj<MU, SB>(f, h);                                                                  \
+         mul_udag(u, h, r);                                                                   \
+         recon<MU, SB, true>(acc[k], r);                                                      \
+      }                                                                                       \
+   }
+      PLQCD_RB_DIR(0)
+      PLQCD_RB_DIR(1)
+      PLQCD_RB_DIR(2)
+      PLQCD_RB_DIR(3)
+#undef PLQCD_RB_DIR
+#pragma unroll
+      for (int k = 0; k < RB; k++)
+         nrm[k] = epilogue<EPI, NORM>(in.at_rhs(gi * RB + k), out.at_rhs(gi * RB + k), a, gi * RB + k, p, j, acc[k],
+                                      (const C *)nullptr);
+   }
+   if (NORM) {
+#pragma unroll
+      for (int k = 0; k < RB; k++)
+         chunk_reduce<BLOCK>(nrm[k], js, gi, G, p * a.nchunk + (blk * a.spb + js) / REDUCE_CHUNK,
+                             j - (js & (REDUCE_CHUNK - 1)) < a.Vh, a.partial + (size_t)(gi * RB + k) * 2 * a.nchunk);
+   }
+}
+
+template <int MODE, bool DAG, int EPI, bool NORM, int BLOCK, int RB, class SIO, class GIO>
+__global__ __launch_bounds__(BLOCK) void hop_rb_kernel(SIO in, SIO out, GIO g, HopArgs a)
+{
+   hop_block_rb<MODE, DAG, EPI, NORM, BLOCK, RB>(in, out, g, a, logical_block(a));
+}
+
+// One application of H on each of several independent lattices (each with its
+// own gauge field and fields) in a single persistent launch: the grid holds as
+// many blocks as are resident at once, and block b walks the concatenated
+// logical blocks b, b + gridDim.x, ...  (every block leaves after a bounded
+// number of iterations).
+template <class SIO, class GIO>
+struct BatchItem {
+   SIO in, out;
+   GIO g;
+   HopArgs a;
+   int first;  // first global block of this item
+   int nb;     // blocks of this item
+};
+
+template <int MODE, int BLOCK, class SIO, class GIO>
+__global__ __launch_bounds__(BLOCK) void hop_batch_kernel(const BatchItem<SIO, GIO> *items, int nitems, int total)
+{
+   for (int b = blockIdx.x; b < total; b += gridDim.x) {
+      int it = 0;
+      while (it + 1 < nitems && b >= items[it + 1].first) it++;
+      const BatchItem<SIO, GIO> &I = items[it];
+      hop_block<MODE, false, 0, false, BLOCK>(I.in, I.out, I.g, I.a, b - I.first);
+   }
+}
+
+// t-march variant (wilson mode, full lattice): a thread owns one (x,y,z)
+// column and walks T consecutive t-slices.  Along t each spinor and each U_t
+// is loaded once: psi(t+1), read as the forward t-neighbour at step t, is kept
+// (it is the centre site of step t+1) and its projection, multiplied by
+// U_t(t+1)^+ at step t+1 (that step's forward link), gives the backward
+// t-term of step t+2.  The spatial directions read their neighbours from L2
+// as the gather kernel does.  The site parity alternates from step to step;
+// the index of the site inside its parity advances by S = L0/2*L1*L2 per
+// t-slice.  Terms are accumulated in the same order as in hop_kernel.
+template <bool DAG, int EPI, bool NORM, int BLOCK, class SIO, class GIO>
+__global__ __launch_bounds__(BLOCK) void hop_tmarch_kernel(SIO in, SIO out, GIO g, HopArgs a)
+{
+   typedef typename SIO::C C;
+   constexpr int SF = DAG ? 1 : -1;  // forward projector sign
+   const int S = a.S;
+   const int nbS = (S + a.spb - 1) / a.spb;
+   int lb = logical_block(a);
+   const int q = lb & 1;  // parity of the column's first site
+   lb >>= 1;
+   const int sb = lb % nbS;
+   const int chunk = lb / nbS;
+   const int r = threadIdx.x % a.R;
+   const int js = threadIdx.x / a.R;
+   const int c = sb * a.spb + js;
+   const bool valid = c < S;
+   const SIO si = in.at_rhs(r), so = out.at_rhs(r);
+
+   const int t0 = chunk * a.T;
+   const int cc = valid ? c : 0;
+   const int row = cc / a.Lxh;  // y + L1*z
+   const int xh = cc - row * a.Lxh;
+   const int y = row % a.L[1];
+   const int z = row / a.L[1];
+   const int x = 2 * xh + ((q + y + z + t0) & 1);
+   int cu[3], cd[3];
+   {
+      const int xp = (x + 1 == a.L[0]) ? 0 : x + 1;
+      const int xm = (x == 0) ? a.L[0] - 1 : x - 1;
+      cu[0] = row * a.Lxh + (xp >> 1);
+      cd[0] = row * a.Lxh + (xm >> 1);
+      const int ryp = (y == a.L[1] - 1) ? row - (a.L[1] - 1) : row + 1;
+      const int rym = (y == 0) ? row + (a.L[1] - 1) : row - 1;
+      const int rzp = (z == a.L[2] - 1) ? row - (a.L[2] - 1) * a.L[1] : row + a.L[1];
+      const int rzm = (z == 0) ? row + (a.L[2] - 1) * a.L[1] : row - a.L[1];
+      cu[1] = ryp * a.Lxh + xh;
+      cd[1] = rym * a.Lxh + xh;
+      cu[2] = rzp * a.Lxh + xh;
+      cd[2] = rzm * a.Lxh + xh;
+   }
+   const int Lt = a.L[3];
+   // prologue: cen = psi(t0); b = U_t(t0-1)^+ (1 -+ g_t) psi(t0-1)
+   C cen[12], b[6];
+   if (valid) {
+      C f[12], u[9], kk[6];
+      si.load(q, c + t0 * S, cen);
+      const int tm = (t0 == 0 ? Lt : t0) - 1;
+      si.load(q ^ 1, c + tm * S, f);
+      proj<3, -SF>(f, kk);
+      g.load(q ^ 1, 3, c + tm * S, u);
+      mul_udag(u, kk, b);
+   }
+   for (int s = 0; s < a.T; s++) {
+      const int t = t0 + s;
+      const int p = q ^ (s & 1);
+      const int base = t * S;
+      const int j = cc + base;
+      double nrm = 0.0;
+      if (valid) {
+         C acc[12];
+#pragma unroll
+         for (int m = 0; m < 12; m++) acc[m] = C{0, 0};
+         hop_fwd<0, PLQCD_HOP_WILSON, DAG>(si, g, p, j, cu[0] + base, acc);
+         hop_bwd<0, DAG>(si, g, p, cd[0] + base, acc);
+         hop_fwd<1, PLQCD_HOP_WILSON, DAG>(si, g, p, j, cu[1] + base, acc);
+         hop_bwd<1, DAG>(si, g, p, cd[1] + base, acc);
+         hop_fwd<2, PLQCD_HOP_WILSON, DAG>(si, g, p, j, cu[2] + base, acc);
+         hop_bwd<2, DAG>(si, g, p, cd[2] + base, acc);
+         C f[12];
+         {
+            C h[6], u[9], rr[6], kk[6];
+            const int tp = (t + 1 == Lt) ? 0 : t + 1;
+            si.load(p ^ 1, c + tp * S, f);
+            proj<3, SF>(f, h);
+            g.load(p, 3, j, u);
+            mul_u(u, h, rr);
+            recon<3, SF, true>(acc, rr);
+            recon<3, -SF, true>(acc, b);
+            proj<3, -SF>(cen, kk);
+            mul_udag(u, kk, b);
+         }
+         nrm = epilogue<EPI, NORM>(si, so, a, r, p, j, acc, cen);
+#pragma unroll
+         for (int m = 0; m < 12; m++) cen[m] = f[m];
+      }
+      if (NORM) chunk_reduce<BLOCK>(nrm, js, r, a.R, p * a.nchunk + (sb * a.spb + js + base) / REDUCE_CHUNK, valid,
+                                    a.partial + (size_t)r * 2 * a.nchunk);
+   }
+}
+
+// kernel variants: gather (1 RHS per lane), t-march, RHS-blocked gather
+enum Variant { V_GATHER = 0, V_MARCH = 1, V_RB2 = 2, V_RB4 = 4 };
+
+template <int MODE, bool DAG, int EPI, bool NORM, int BLOCK, class S, class G>
+void launch_one(int variant, dim3 grid, dim3 block, hipStream_t st, S si, S so, G gg, const HopArgs &a)
+{
+   if (variant == V_MARCH) {
+      if constexpr (MODE == PLQCD_HOP_WILSON)
+         hipLaunchKernelGGL((hop_tmarch_kernel<DAG, EPI, NORM, BLOCK, S, G>), grid, block, 0, st, si, so, gg, a);
+   } else if (variant == V_RB2) {
+      hipLaunchKernelGGL((hop_rb_kernel<MODE, DAG, EPI, NORM, BLOCK, 2, S, G>), grid, block, 0, st, si, so, gg, a);
+   } else if (variant == V_RB4) {
+      hipLaunchKernelGGL((hop_rb_kernel<MODE, DAG, EPI, NORM, BLOCK, 4, S, G>), grid, block, 0, st, si, so, gg, a);
+   } else {
+      hipLaunchKernelGGL((hop_kernel<MODE, DAG, EPI, NORM, BLOCK, S, G>), grid, block, 0, st, si, so, gg, a);
+   }
+}
+
+template <int MODE, bool DAG, int EPI, bool NORM, class S, class G>
+void launch_block(int maxblock, int variant, dim3 grid, dim3 block, hipStream_t st, S si, S so, G gg,
+                  const HopArgs &a)
+{
+   if (maxblock <= 128) launch_one<MODE, DAG, EPI, NORM, 128>(variant, grid, block, st, si, so, gg, a);
+   else if (maxblock <= 256) launch_one<MODE, DAG, EPI, NORM, 256>(variant, grid, block, st, si, so, gg, a);
+   else launch_one<MODE, DAG, EPI, NORM, 512>(variant, grid, block, st, si, so, gg, a);
+}
+
+template <int PREC, int LINKS>
+int launch_t(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip_field *out, int parity, int mode,
+             const plqcd::DslashOp &op)
+{
+   const plqcd_hip_lattice *lat = g->lat;
+   HopArgs a = make_hop_args(lat, parity);
+   typedef SpinorIO<PREC> S;
+   typedef GaugeIO<PREC, LINKS> G;
+   const S si = spinor_io<PREC>(in), so = spinor_io<PREC>(out);
+   const G gg = gauge_io<PREC, LINKS>(g);
+   const int R = in->nrhs;
+   HopTuning tu = g_tuning[PREC];
+   plqcd::lookup_tuning(plqcd::TuneKey{{lat->L[0], lat->L[1], lat->L[2], lat->L[3]}, PREC, LINKS, in->nrhs}, &tu);
+   // RHS per lane (RHS-blocked gather) when it divides R
+   const int rb = (tu.rb == 2 || tu.rb == 4) && R % tu.rb == 0 ? tu.rb : 1;
+   const int lanes = R / rb;  // lanes per site
+   // block shape: `lanes` lanes per site and a whole number of reduction chunks
+   int target = tu.block;
+   if (lanes * REDUCE_CHUNK > target) {
+      target = 128;
+      while (target < lanes * REDUCE_CHUNK) target *= 2;
+   }
+   if (target > 512)
+      return plqcd::set_error(2, "hop: %d right-hand sides exceed the per-launch limit of %d", R, 512 / REDUCE_CHUNK);
+   a.R = R;
+   a.spb = REDUCE_CHUNK * (target / (REDUCE_CHUNK * lanes));
+   const int threads = lanes * a.spb;
+   const int maxblock = threads <= 128 ? 128 : threads <= 256 ? 256 : 512;
+   a.kappa = op.kappa;
+   a.partial = op.partial;
+   a.nchunk = plqcd::reduce_chunks(lat->Vh);
+   const bool dag = op.dag != 0;
+   const int epi = op.kappa ? 1 : 0;
+   const bool norm = op.partial != nullptr;
+   if ((dag || epi || norm) && mode != PLQCD_HOP_WILSON)
+      return plqcd::set_error(2, "hop: M, M^+ and norms need the wilson mode");
+   if (norm && a.nparity != 2) return plqcd::set_error(2, "hop: norms are only fused into full-lattice launches");
+   const int T = tu.tmarch;
+   // the t-march kernel serves full-lattice wilson applications whose time
+   // extent is a multiple of T (and, with a norm, whose t-slices hold whole
+   // reduction chunks)
+   const bool march = rb == 1 && T > 0 && mode == PLQCD_HOP_WILSON && a.nparity == 2 && lat->L[3] % T == 0 &&
+                      (!norm || a.S % REDUCE_CHUNK == 0);
+   const int variant = march ? V_MARCH : (rb == 2 ? V_RB2 : rb == 4 ? V_RB4 : V_GATHER);
+   a.T = march ? T : 1;
+   const int nblk = march ? ((a.S + a.spb - 1) / a.spb) * (lat->L[3] / T) : (lat->Vh + a.spb - 1) / a.spb;
+   const int nb = nblk * a.nparity;
+   // the remapped orders need whole rounds of 8 XCDs (and of 8*group blocks)
+   a.order = tu.order;
+   a.group = tu.group > 0 ? tu.group : 1;
+   if ((a.order == 1 && nb % 8) || (a.order == 2 && nb % (8 * a.group))) a.order = 0;
+   const dim3 grid(nb), block(threads);
+   hipStream_t st = lat->stream;
+   if (mode == PLQCD_HOP_COMPAT) {
+      launch_block<PLQCD_HOP_COMPAT, false, 0, false>(maxblock, variant, grid, block, st, si, so, gg, a);
+   } else if (!dag) {
+      if (epi == 0) launch_block<PLQCD_HOP_WILSON, false, 0, false>(maxblock, variant, grid, block, st, si, so, gg, a);
+      else if (!norm) launch_block<PLQCD_HOP_WILSON, false, 1, false>(maxblock, variant, grid, block, st, si, so, gg, a);
+      else launch_block<PLQCD_HOP_WILSON, false, 1, true>(maxblock, variant, grid, block, st, si, so, gg, a);
+   } else {
+      if (epi == 0) launch_block<PLQCD_HOP_WILSON, true, 0, false>(maxblock, variant, grid, block, st, si, so, gg, a);
+      else if (!norm) launch_block<PLQCD_HOP_WILSON, true, 1, false>(maxblock, variant, grid, block, st, si, so, gg, a);
+      else launch_block<PLQCD_HOP_WILSON, true, 1, true>(maxblock, variant, grid, block, st, si, so, gg, a);
+   }
+   PLQCD_HIP_TRY(hipGetLastError());
+   return 0;
+}
+
+
+template <int PREC, int LINKS>
+int launch_batch_t(int n, const plqcd_hip_gauge *const *g, const plqcd_hip_field *const *in,
+                   plqcd_hip_field *const *out, int mode)
+{
+   typedef SpinorIO<PREC> S;
+   typedef GaugeIO<PREC, LINKS> G;
+   std::vector<BatchItem<S, G>> items(n);
+   int total = 0, maxR = 1;
+   for (int i = 0; i < n; i++) maxR = std::max(maxR, in[i]->nrhs);
+   if (maxR * REDUCE_CHUNK > 256) return plqcd::set_error(2, "hop_batch: at most 32 right-hand sides per item");
+   for (int i = 0; i < n; i++) {
+      BatchItem<S, G> &I = items[i];
+      I.in = spinor_io<PREC>(in[i]);
+      I.out = spinor_io<PREC>(out[i]);
+      I.g = gauge_io<PREC, LINKS>(g[i]);
+      I.a = make_hop_args(g[i]->lat, PLQCD_PARITY_ALL);
+      I.a.R = in[i]->nrhs;
+      I.a.spb = REDUCE_CHUNK * (256 / (REDUCE_CHUNK * I.a.R));
+      I.nb = 2 * ((I.a.Vh + I.a.spb - 1) / I.a.spb);
+      I.first = total;
+      total += I.nb;
+   }
+   hipStream_t st = g[0]->lat->stream;
+   BatchItem<S, G> *d = nullptr;
+   PLQCD_HIP_TRY(hipMallocAsync((void **)&d, n * sizeof(BatchItem<S, G>), st));
+   hipError_t e = hipMemcpyAsync(d, items.data(), n * sizeof(BatchItem<S, G>), hipMemcpyHostToDevice, st);
+   int rc = 0;
+   if (e == hipSuccess) {
+      int dev = 0, ncu = 0, per = 0;
+      (void)hipGetDevice(&dev);
+      (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+      if (mode == PLQCD_HOP_COMPAT) {
+         (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, hop_batch_kernel<PLQCD_HOP_COMPAT, 256, S, G>, 256, 0);
+         const int grid = std::max(1, std::min(total, std::max(per, 1) * ncu));
+         hipLaunchKernelGGL((hop_batch_kernel<PLQCD_HOP_COMPAT, 256, S, G>), dim3(grid), dim3(256), 0, st, d, n, total);
+      } else {
+         (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, hop_batch_kernel<PLQCD_HOP_WILSON, 256, S, G>, 256, 0);
+         const int grid = std::max(1, std::min(total, std::max(per, 1) * ncu));
+         hipLaunchKernelGGL((hop_batch_kernel<PLQCD_HOP_WILSON, 256, S, G>), dim3(grid), dim3(256), 0, st, d, n, total);
+      }
+      e = hipGetLastError();
+   }
+   if (e != hipSuccess) rc = plqcd::set_error(10, "hop_batch: %s", hipGetErrorString(e));
+   (void)hipFreeAsync(d, st);
+   return rc;
+}
+
+}  // namespace
+
+// Entry points of one (precision, link format), explicitly instantiated in
+// dslash_<precision>_l<links>.hip.
+template <int PREC, int LINKS>
+int plqcd::hop_launch(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip_field *out, int parity, int mode,
+                      const DslashOp &op)
+{
+   return launch_t<PREC, LINKS>(g, in, out, parity, mode, op);
+}
+
+template <int PREC, int LINKS>
+int plqcd::hop_batch_launch(int n, const plqcd_hip_gauge *const *g, const plqcd_hip_field *const *in,
+                            plqcd_hip_field *const *out, int mode)
+{
+   return launch_batch_t<PREC, LINKS>(n, g, in, out, mode);
+}

@@ -3,7 +3,8 @@
 // For one (lattice, storage precision, link format, RHS count) it times every
 // candidate launch shape -- block->tile order (plain, contiguous per XCD, XCD
 // runs of 8 or 16 tiles), threads per block, and t-march length (gather
-// kernel or T t-slices per thread, T dividing the time extent) -- with HIP
+// kernel or T t-slices per thread, T dividing the time extent), and right-
+// hand sides per lane (RHS-blocked gather) -- with HIP
 // events on the lattice stream, keeps the fastest in the per-shape cache that
 // every later hopping launch of that shape consults, and returns it.
 #include <vector>
@@ -11,7 +12,7 @@
 #include "dslash.h"
 
 extern "C" int plqcd_hip_autotune(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip_field *out,
-                                  int reps, int best[4], double *best_ms)
+                                  int reps, int best[5], double *best_ms)
 {
    plqcd::clear_error();
    if (!g || !in || !out || reps < 1) return plqcd::set_error(1, "autotune: bad arguments");
@@ -25,6 +26,9 @@ extern "C" int plqcd_hip_autotune(const plqcd_hip_gauge *g, const plqcd_hip_fiel
    std::vector<int> marches = {0};
    for (int T : {2, 4, 8, 16, 32})
       if (lat->L[3] % T == 0) marches.push_back(T);
+   std::vector<int> rbs = {1};
+   for (int rb : {2, 4})
+      if (in->nrhs % rb == 0) rbs.push_back(rb);
    HopTuning prev{};
    const bool had = plqcd::lookup_tuning(key, &prev);
    double best_t = 1e300;
@@ -33,10 +37,12 @@ extern "C" int plqcd_hip_autotune(const plqcd_hip_gauge *g, const plqcd_hip_fiel
    PLQCD_HIP_TRY(hipEventCreate(&e0));
    PLQCD_HIP_TRY(hipEventCreate(&e1));
    int rc = 0;
+   for (int rb : rbs)
    for (int T : marches)
       for (int blk : blocks)
          for (const Cand &o : orders) {
-            const HopTuning c{o.order, o.group, blk, T};
+            if (rb > 1 && T > 0) continue;  // the t-march kernel runs one RHS per lane
+            const HopTuning c{o.order, o.group, blk, T, rb};
             plqcd::store_tuning(key, c);
             rc = plqcd_hip_hop(g, in, out, PLQCD_PARITY_ALL, PLQCD_HOP_WILSON);  // warm-up
             if (rc) break;
@@ -65,6 +71,7 @@ extern "C" int plqcd_hip_autotune(const plqcd_hip_gauge *g, const plqcd_hip_fiel
       best[1] = best_c.group;
       best[2] = best_c.block;
       best[3] = best_c.tmarch;
+      best[4] = best_c.rb;
    }
    if (best_ms) *best_ms = best_t;
    return 0;

@@ -229,8 +229,8 @@ def test_benchmark_shape_32c64_f32_error(hp, orc):
 
 # ---------------------------------------------------------- launch shapes
 
-TUNINGS = [(0, 1, 256, 0), (1, 1, 128, 0), (2, 4, 64, 0), (2, 16, 512, 0),
-           (2, 16, 256, 2), (1, 1, 128, 4), (2, 8, 64, 8), (0, 1, 256, 16)]
+TUNINGS = [(0, 1, 256, 0, 1), (1, 1, 128, 0, 1), (2, 4, 64, 0, 1), (2, 16, 512, 0, 1),
+           (2, 16, 256, 2, 1), (1, 1, 128, 4, 1), (2, 8, 64, 8, 1), (0, 1, 256, 16, 1)]
 
 
 @pytest.mark.parametrize("tuning", TUNINGS)
@@ -243,7 +243,7 @@ def test_launch_shapes_match_oracle(hp, orc, tuning, prec, links, tol):
     q, u = inputs(hp, L)
     ref = orc.hopping_full(q, u, L, orc.MODE_WILSON)
     lib = capi.lib()
-    saved = (C.c_int * 4)()
+    saved = (C.c_int * 5)()
     lib.plqcd_hip_hop_get_tuning(prec, saved)
     assert lib.plqcd_hip_hop_tuning(prec, *tuning) == 0
     try:
@@ -253,3 +253,64 @@ def test_launch_shapes_match_oracle(hp, orc, tuning, prec, links, tol):
             assert site_rel_err(out, r) < tol, (tuning, mode)
     finally:
         lib.plqcd_hip_hop_tuning(prec, *saved)
+
+
+# ------------------------------------------------- RHS-blocked gather kernel
+
+def _set_tuning(prec, t):
+    from plqcd_amd import capi
+    assert capi.lib().plqcd_hip_hop_tuning(prec, *t) == 0
+
+
+def _get_tuning(prec):
+    import ctypes as C
+    from plqcd_amd import capi
+    saved = (C.c_int * 5)()
+    assert capi.lib().plqcd_hip_hop_get_tuning(prec, saved) == 0
+    return tuple(saved)
+
+
+@pytest.mark.parametrize("R", [4, 12])
+@pytest.mark.parametrize("prec,links", [(0, 18), (1, 12), (1, 18), (2, 12)])
+def test_rhs_blocked_bitwise_equal_single_rhs(hp, prec, links, R):
+    """Every RHS of an RHS-blocked launch (2 or 4 RHS per thread) is bitwise
+    the RHS's own one-RHS-per-thread result (gather and t-march kernels), in
+    both modes and on each parity."""
+    L = (8, 4, 6, 8)
+    V = int(np.prod(L))
+    _, u = inputs(hp, L)
+    qs = [hp.random_spinor(V, 300 + r) for r in range(R)]
+    lat = hp.Lattice(L)
+    g = hp.Gauge(lat, prec, links).upload(u)
+    fi, fo = hp.Field(lat, prec, R), hp.Field(lat, prec, R)
+    f1i, f1o = hp.Field(lat, prec), hp.Field(lat, prec)
+    for r, q in enumerate(qs):
+        fi.upload(q, r)
+    saved = _get_tuning(prec)
+    try:
+        for mode in (hp.HOP_WILSON, hp.HOP_COMPAT):
+            for parity in (hp.PARITY_ALL, hp.PARITY_EVEN, hp.PARITY_ODD):
+                singles = []
+                for tm in (0, 4):
+                    _set_tuning(prec, (1, 1, 256, tm, 1))
+                    outs = []
+                    for q in qs:
+                        f1i.upload(q)
+                        f1o.upload(np.zeros_like(q))
+                        hp.hop(g, f1i, f1o, parity, mode)
+                        outs.append(f1o.download())
+                    singles.append(outs)
+                for r in range(R):
+                    assert np.array_equal(singles[0][r], singles[1][r]), ("t-march", mode, parity, r)
+                for rb, blk in ((2, 128), (4, 256), (4, 64)):
+                    _set_tuning(prec, (2, 4, blk, 0, rb))
+                    for r in range(R):
+                        fo.upload(np.zeros_like(qs[0]), r)
+                    hp.hop(g, fi, fo, parity, mode)
+                    for r in range(R):
+                        assert np.array_equal(fo.download(r), singles[0][r]), (rb, blk, mode, parity, r)
+    finally:
+        _set_tuning(prec, saved)
+        for f in (fi, fo, f1i, f1o):
+            f.close()
+        g.close(); lat.close()

@@ -77,6 +77,40 @@ def test_cg_batch_equals_alone_bitwise(hp):
             assert np.array_equal(x1.download(), x.download(r))
 
 
+def test_cg_rhs_blocked_equals_alone_bitwise(hp):
+    """A 4-RHS solve whose hopping kernels run 4 RHS per thread (norm and
+    axpy epilogues included) follows each RHS's lone iterates bitwise."""
+    import ctypes as C
+    from plqcd_amd import capi
+    from plqcd_amd import solver as sv
+    lib = capi.lib()
+    L = (8, 4, 4, 8)
+    kaps, tols = [0.12, 0.124, 0.1245, 0.11], [1e-8, 1e-10, 1e-12, 1e-9]
+    lat, g, b, u, bs = setup(hp, L, nrhs=4)
+    gs = hp.Gauge(lat, 2, 12).upload(u)
+    saved = []
+    for prec in range(3):
+        t = (C.c_int * 5)()
+        lib.plqcd_hip_hop_get_tuning(prec, t)
+        saved.append(tuple(t))
+    try:
+        assert lib.plqcd_hip_hop_tuning(-1, 2, 8, 128, 0, 4) == 0
+        for sloppy in (None, gs):
+            x = hp.Field(lat, nrhs=4)
+            st = sv.cg_solve(g, b, x, kaps, tols, sloppy=sloppy, delta=0.1 if sloppy else 0.0)
+            for r in range(4):
+                b1 = hp.Field(lat).upload(bs[r])
+                x1 = hp.Field(lat)
+                st1 = sv.cg_solve(g, b1, x1, kaps[r], tols[r], sloppy=sloppy, delta=0.1 if sloppy else 0.0)[0]
+                assert st1["iterations"] == st[r]["iterations"]
+                assert st1["true_residual"] == st[r]["true_residual"]
+                assert np.array_equal(x1.download(), x.download(r))
+                assert st[r]["converged"]
+    finally:
+        for prec in range(3):
+            lib.plqcd_hip_hop_tuning(prec, *saved[prec])
+
+
 def test_cg_graph_replay_equals_eager(hp):
     from plqcd_amd import solver as sv
     L = (4, 4, 8, 8)
