@@ -246,30 +246,32 @@ def solver_suite():
         for r in range(R):
             fi.upload(hp.random_spinor(V, 10 + r), r)
         t_def = time_hop(hp, g, fi, fo)
-        best = (C.c_int * 5)()
+        best = (C.c_int * 7)()
         bms = C.c_double()
         capi.check(lib.plqcd_hip_autotune(g.h, fi.h, fo.h, 5, best, C.byref(bms)), "autotune")
         t_tuned = time_hop(hp, g, fi, fo)
         emit({"suite": "solver", "case": f"autotune_32c64_{PREC_NAMES[prec]}_l{links}_R{R}",
               "default_us": round(t_def * 1e3, 1), "tuned_us": round(t_tuned * 1e3, 1),
               "best": {"order": best[0], "group": best[1], "block": best[2], "tmarch": best[3],
-                       "rb": best[4]}})
+                       "rb": best[4], "tile_y": best[5], "tile_z": best[6]}})
         fi.close(); fo.close(); g.close()
     lib.plqcd_hip_autotune_clear()
     lat.close()
 
 
 def calib_suite():
-    """HBM probes of known byte counts (copy 1 GiB, read 1 GiB, write 1 GiB),
-    then two applications of H per storage format at 32^3 x 64 with the default
-    launch shapes: one run holds the counter calibration and the kernels."""
+    """HBM probes of known byte counts (copy 1 GiB, read 1 GiB, write 1 GiB;
+    one untimed and 10 timed launches each), then two applications of H per
+    storage format at 32^3 x 64 with the default launch shapes: one run holds
+    the counter calibration and the kernels."""
     import ctypes as C
     from plqcd_amd import capi, hopping as hp
     lib = capi.lib()
     for kind, name in ((0, "copy"), (1, "read"), (2, "write")):
         gbs = C.c_double()
-        capi.check(lib.plqcd_hip_stream_bandwidth(kind, 1 << 30, 1, C.byref(gbs)), "stream_bandwidth")
-        emit({"suite": "calib", "case": f"stream_{name}_1GiB", "bytes_per_buffer": 1 << 30, "GBps": round(gbs.value, 1)})
+        capi.check(lib.plqcd_hip_stream_bandwidth(kind, 1 << 30, 10, C.byref(gbs)), "stream_bandwidth")
+        emit({"suite": "calib", "case": f"stream_{name}_1GiB", "bytes_per_buffer": 1 << 30, "timed_launches": 10,
+              "GBps": round(gbs.value, 1)})
     L = (32, 32, 32, 64)
     V = int(np.prod(L))
     u = hp.random_gauge_su3(V, 1)

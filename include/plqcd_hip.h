@@ -304,19 +304,29 @@ void plqcd_hip_server_destroy(plqcd_hip_server *s);
  *   t-slices of a column), otherwise the one-site-per-thread gather kernel;
  *   rb (1, 2 or 4): right-hand sides per thread of the gather kernel when it
  *   divides the field's RHS count (links are loaded once per rb RHS; the
- *   t-march kernel is used only with rb 1).
+ *   t-march kernel is used only with rb 1);
+ *   tile_y, tile_z > 0: one-RHS t-march launches use the LDS-tiled kernel,
+ *   whose blocks own tiles of L0 x tile_y x tile_z sites (both parities) and
+ *   exchange neighbour spinors through LDS, when the tile divides the lattice
+ *   and has 128 or 256 (or, below F64, 512) lanes = L0 tile_y tile_z
+ *   (block is then ignored); 0, 0: no tile.
  * Each RHS's result is bitwise the same for every launch shape.  Returns 0,
  * or nonzero for bad values. */
-int plqcd_hip_hop_tuning(int prec, int order, int group, int block, int tmarch, int rb);
-/* Current launch shape of precision `prec`: out = {order, group, block, tmarch, rb}. */
-int plqcd_hip_hop_get_tuning(int prec, int out[5]);
+int plqcd_hip_hop_tuning(int prec, int order, int group, int block, int tmarch, int rb, int tile_y, int tile_z);
+/* Back to the built-in launch shapes of precision `prec` (-1: all), which
+ * differ for one and for several right-hand sides; a shape set with
+ * plqcd_hip_hop_tuning applies to every RHS count until reset. */
+int plqcd_hip_hop_tuning_reset(int prec);
+/* Current launch shape of precision `prec` (one RHS):
+ * out = {order, group, block, tmarch, rb, tile_y, tile_z}. */
+int plqcd_hip_hop_get_tuning(int prec, int out[7]);
 /* Times every launch shape for the shape of (g, in) -- lattice, precision,
  * link format, RHS count -- with `reps` timed applications each, and makes
  * the fastest the launch shape of that shape from now on.  best = {order,
- * group, block, tmarch, rb}, *best_ms = its time per application.  in/out
- * are overwritten. */
+ * group, block, tmarch, rb, tile_y, tile_z}, *best_ms = its time per
+ * application.  in/out are overwritten. */
 int plqcd_hip_autotune(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip_field *out, int reps,
-                       int best[5], double *best_ms);
+                       int best[7], double *best_ms);
 /* Forgets every autotuned shape (back to the per-precision defaults). */
 int plqcd_hip_autotune_clear(void);
 

@@ -101,8 +101,9 @@ _PROTOS = {
     "plqcd_hip_autotune": (C.c_int, [vp, vp, vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     "plqcd_hip_autotune_clear": (C.c_int, []),
     "plqcd_hip_hop_batch": (C.c_int, [C.c_int, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.c_int]),
-    "plqcd_hip_hop_tuning": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "plqcd_hip_hop_tuning": (C.c_int, [C.c_int] * 8),
     "plqcd_hip_hop_get_tuning": (C.c_int, [C.c_int, C.POINTER(C.c_int)]),
+    "plqcd_hip_hop_tuning_reset": (C.c_int, [C.c_int]),
     "plqcd_hip_stream_bandwidth": (C.c_int, [C.c_int, C.c_size_t, C.c_int, C.POINTER(C.c_double)]),
     "plqcd_random_gauge_su3": (None, [vp, C.c_int, C.c_uint64]),
     "plqcd_random_spinor_gauss": (None, [vp, C.c_int, C.c_uint64]),

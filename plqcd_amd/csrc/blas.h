@@ -9,11 +9,8 @@
 namespace plqcd {
 
 size_t partial_doubles(const plqcd_hip_field *f);
-int blas_norm_finish(const plqcd_hip_field *f, const double *partial, double *out);
+int blas_norm_finish(const plqcd_hip_field *f, double *partial, double *out);  // overwrites partial
 
-// x += alpha p ; r -= alpha Ap ; partial <- |r|^2   (all one precision)
-int blas_cg_update(plqcd_hip_field *x, const plqcd_hip_field *p, plqcd_hip_field *r, const plqcd_hip_field *ap,
-                   const double *alpha, double *partial);
 // p = r + beta p
 int blas_xpay(const plqcd_hip_field *r, const double *beta, plqcd_hip_field *p);
 // The masked kernels below touch only the RHS r with mask[r] != 0 (all when

@@ -32,61 +32,6 @@ __device__ __forceinline__ void blas_reduce(const BlasArgs &a, double v, int p, 
    chunk_reduce<kBlock>(v, js, r, a.R, p * a.nchunk + j / REDUCE_CHUNK, j < a.Vh, a.partial + (size_t)r * 2 * a.nchunk);
 }
 
-template <int P>
-__device__ __forceinline__ void load_d(const SpinorIO<P> &f, int par, int j, d2 s[12])
-{
-   typename SpinorIO<P>::C t[12];
-   f.load(par, j, t);
-#pragma unroll
-   for (int c = 0; c < 12; c++) s[c] = d2{(double)t[c].x, (double)t[c].y};
-}
-
-template <int P>
-__device__ __forceinline__ void store_d(const SpinorIO<P> &f, int par, int j, const d2 s[12])
-{
-   typedef typename SpinorIO<P>::C C;
-   typedef typename SpinorIO<P>::real real;
-   C t[12];
-#pragma unroll
-   for (int c = 0; c < 12; c++) t[c] = C{(real)s[c].x, (real)s[c].y};
-   f.store(par, j, t);
-}
-
-__device__ __forceinline__ double norm2(const d2 s[12])
-{
-   double n = 0.0;
-#pragma unroll
-   for (int c = 0; c < 12; c++) n = fma(s[c].y, s[c].y, fma(s[c].x, s[c].x, n));
-   return n;
-}
-
-// x += alpha p ; r -= alpha Ap ; |r|^2
-template <int P>
-__global__ __launch_bounds__(kBlock) void k_cg_update(SpinorIO<P> x, SpinorIO<P> pv, SpinorIO<P> rv, SpinorIO<P> ap,
-                                                      const double *alpha, BlasArgs a)
-{
-   int p, j, js, r;
-   double n = 0.0;
-   if (blas_site(a, p, j, js, r)) {
-      const double al = alpha[r];
-      const SpinorIO<P> X = x.at_rhs(r), Pp = pv.at_rhs(r), Rr = rv.at_rhs(r), A = ap.at_rhs(r);
-      d2 xs[12], ps[12], rs[12], as[12];
-      load_d(X, p, j, xs);
-      load_d(Pp, p, j, ps);
-      load_d(Rr, p, j, rs);
-      load_d(A, p, j, as);
-#pragma unroll
-      for (int c = 0; c < 12; c++) {
-         xs[c] = d2{fma(al, ps[c].x, xs[c].x), fma(al, ps[c].y, xs[c].y)};
-         rs[c] = d2{fma(-al, as[c].x, rs[c].x), fma(-al, as[c].y, rs[c].y)};
-      }
-      store_d(X, p, j, xs);
-      store_d(Rr, p, j, rs);
-      n = norm2(rs);
-   }
-   if (a.partial) blas_reduce(a, n, p, j, js, r);
-}
-
 // p = r + beta p
 template <int P>
 __global__ __launch_bounds__(kBlock) void k_xpay(SpinorIO<P> rv, const double *beta, SpinorIO<P> pv, BlasArgs a)
@@ -191,21 +136,46 @@ __global__ __launch_bounds__(kBlock) void k_xmcy_parity(SpinorIO<P> xv, SpinorIO
    store_d(Y, p, j, ys);
 }
 
-// out[r] = sum partial[r*stride + first + k], k < n, in a fixed order
-__global__ __launch_bounds__(256) void k_reduce(const double *partial, size_t stride, int first, int n, double *out)
+// out[r] = sum of partial[r*stride + first + k], k < n, in an order fixed by n
+// alone.  Stage 1: block (b, r) sums segment b of kSeg values (thread t adds
+// elements t, t+256, ... of the segment, then a block tree) and stores the sum
+// over the segment's first element, which no other block reads.  Stage 2: one
+// block per RHS sums the segment results the same way.
+constexpr int kSeg = 4096;
+
+__device__ __forceinline__ double block_sum256(double s)
 {
    __shared__ double red[256];
-   const int r = blockIdx.x, t = threadIdx.x;
-   const double *p = partial + r * stride + first;
-   double s = 0.0;
-   for (int k = t; k < n; k += 256) s += p[k];
+   const int t = threadIdx.x;
    red[t] = s;
    __syncthreads();
    for (int w = 128; w > 0; w >>= 1) {
       if (t < w) red[t] += red[t + w];
       __syncthreads();
    }
-   if (t == 0) out[r] = red[0];
+   return red[0];
+}
+
+__global__ __launch_bounds__(256) void k_reduce_seg(double *partial, size_t stride, int first, int n)
+{
+   const int b = blockIdx.x, r = blockIdx.y, t = threadIdx.x;
+   double *p = partial + r * stride + first + (size_t)b * kSeg;
+   const int m = min(kSeg, n - b * kSeg);
+   double s = 0.0;
+   for (int k = t; k < m; k += 256) s += p[k];
+   const double tot = block_sum256(s);  // every read of the segment precedes this
+   if (t == 0) p[0] = tot;
+}
+
+__global__ __launch_bounds__(256) void k_reduce_fin(const double *partial, size_t stride, int first, int nseg,
+                                                   double *out)
+{
+   const int r = blockIdx.x, t = threadIdx.x;
+   const double *p = partial + r * stride + first;
+   double s = 0.0;
+   for (int k = t; k < nseg; k += 256) s += p[(size_t)k * kSeg];
+   const double tot = block_sum256(s);
+   if (t == 0) out[r] = tot;
 }
 
 BlasArgs make_args(const plqcd_hip_field *f, double *partial)
@@ -253,34 +223,22 @@ dim3 block_of(const BlasArgs &a) { return dim3(a.R * a.spb); }
 
 namespace plqcd {
 
-int reduce_partials(const double *partial, size_t stride, int first, int n, int R, double *out, hipStream_t st)
+int reduce_partials(double *partial, size_t stride, int first, int n, int R, double *out, hipStream_t st)
 {
-   hipLaunchKernelGGL(k_reduce, dim3(R), dim3(256), 0, st, partial, stride, first, n, out);
+   if (n < 1 || R < 1) return set_error(1, "reduce_partials: empty reduction");
+   const int nseg = (n + kSeg - 1) / kSeg;
+   hipLaunchKernelGGL(k_reduce_seg, dim3(nseg, R), dim3(256), 0, st, partial, stride, first, n);
+   hipLaunchKernelGGL(k_reduce_fin, dim3(R), dim3(256), 0, st, partial, stride, first, nseg, out);
    PLQCD_HIP_TRY(hipGetLastError());
    return 0;
 }
 
 size_t partial_doubles(const plqcd_hip_field *f) { return (size_t)f->nrhs * 2 * reduce_chunks(f->lat->Vh); }
 
-int blas_norm_finish(const plqcd_hip_field *f, const double *partial, double *out)
+int blas_norm_finish(const plqcd_hip_field *f, double *partial, double *out)
 {
    const int nc = reduce_chunks(f->lat->Vh);
    return reduce_partials(partial, (size_t)2 * nc, 0, 2 * nc, f->nrhs, out, f->lat->stream);
-}
-
-int blas_cg_update(plqcd_hip_field *x, const plqcd_hip_field *p, plqcd_hip_field *r, const plqcd_hip_field *ap,
-                   const double *alpha, double *partial)
-{
-   if (int rc = check_pair(x, p) | check_pair(x, r) | check_pair(x, ap)) return rc;
-   if (x->prec != p->prec || x->prec != r->prec || x->prec != ap->prec)
-      return set_error(1, "blas_cg_update: precision mismatch");
-   const BlasArgs a = make_args(x, partial);
-   hipStream_t st = x->lat->stream;
-   return dispatch1(x->prec, [&](auto P) {
-      constexpr int PP = decltype(P)::value;
-      hipLaunchKernelGGL(k_cg_update<PP>, grid_of(a), block_of(a), 0, st, spinor_io<PP>(x), spinor_io<PP>(p),
-                         spinor_io<PP>(r), spinor_io<PP>(ap), alpha, a);
-   });
 }
 
 int blas_xpay(const plqcd_hip_field *r, const double *beta, plqcd_hip_field *p)

@@ -16,6 +16,7 @@ struct HopArgs {
    int T;        // t-slices per thread (t-march)
    int R;        // right-hand sides: lanes per site
    int spb;      // sites per block
+   int ty, tz;   // tile extents in y and z (LDS-tiled kernel)
    int nchunk;   // reduction chunks per parity
    const double *kappa;  // per-RHS kappa of M = 1 - kappa H (EPI = 1)
    double *partial;      // [R][2*nchunk] chunk sums of |out|^2 (NORM)
@@ -26,8 +27,16 @@ struct HopTuning {
    int order, group, block;
    int tmarch;  // t-slices per thread of the t-march kernel (0: gather kernel)
    int rb;      // right-hand sides per lane (1, 2 or 4; RHS-blocked gather when > 1)
+   int ty, tz;  // LDS tile of the t-march (y and z extents; 0: untiled)
 };
 extern HopTuning g_tuning[3];  // per storage precision
+
+namespace plqcd {
+// Launch shape for `R` right-hand sides before the per-shape cache: the
+// precision's shape set by plqcd_hip_hop_tuning, else a built-in default for
+// R = 1 or for R > 1.
+HopTuning default_tuning(int prec, int R);
+}  // namespace plqcd
 
 HopArgs make_hop_args(const plqcd_hip_lattice *lat, int parity);
 
@@ -37,10 +46,16 @@ namespace plqcd {
 //   kappa   != null: out = in - kappa[r] * (H in)  (device array, one per RHS)
 //   dag     != 0:    H is replaced by H^+ = gamma5 H gamma5
 //   partial != null: chunk sums of |out|^2 into partial[r][2*nchunk]
+//   alpha   != null: CG update (with dag, kappa and partial): Ap = in - kappa H^+ in
+//                    is not stored; fuse_x += alpha[r] fuse_p and out -= alpha[r] Ap
+//                    (out holds the residual), whose |out|^2 goes to partial
 struct DslashOp {
    int dag = 0;
    const double *kappa = nullptr;
    double *partial = nullptr;
+   const double *alpha = nullptr;
+   const plqcd_hip_field *fuse_p = nullptr;
+   plqcd_hip_field *fuse_x = nullptr;
 };
 
 int dslash(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip_field *out, int parity, int mode,

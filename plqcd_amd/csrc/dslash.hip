@@ -7,8 +7,22 @@
 
 #include "dslash.h"
 
-// Defaults per storage precision (F64, F32, F16), from sweeps at 32^3 x 64.
-HopTuning g_tuning[3] = {{1, 1, 256, 16, 1}, {1, 1, 256, 16, 1}, {1, 1, 64, 4, 1}};
+// Defaults per storage precision (F64, F32, F16), from sweeps at 32^3 x 64:
+// one RHS, and several RHS per launch (R lanes per site favour the gather
+// kernel for 16-bit storage and XCDs sweeping the lattice together).
+namespace {
+const HopTuning kOneRhs[3] = {{1, 1, 256, 16, 1}, {1, 1, 256, 16, 1}, {1, 1, 64, 4, 1}};
+const HopTuning kMultiRhs[3] = {{2, 8, 256, 16, 1}, {2, 8, 256, 4, 1}, {2, 8, 256, 0, 1}};
+bool g_tuning_set[3] = {false, false, false};  // set by plqcd_hip_hop_tuning
+}  // namespace
+
+HopTuning g_tuning[3] = {kOneRhs[0], kOneRhs[1], kOneRhs[2]};
+
+HopTuning plqcd::default_tuning(int prec, int R)
+{
+   if (R <= 1 || g_tuning_set[prec]) return g_tuning[prec];
+   return kMultiRhs[prec];
+}
 
 namespace {
 std::mutex g_tune_mu;
@@ -90,6 +104,12 @@ int plqcd::dslash(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip
    if (in->nrhs != out->nrhs) return plqcd::set_error(1, "hop: RHS count mismatch");
    if (parity < PLQCD_PARITY_EVEN || parity > PLQCD_PARITY_ALL) return plqcd::set_error(1, "hop: bad parity %d", parity);
    if (mode != PLQCD_HOP_COMPAT && mode != PLQCD_HOP_WILSON) return plqcd::set_error(1, "hop: bad mode %d", mode);
+   if (op.alpha) {
+      for (const plqcd_hip_field *f : {op.fuse_p, static_cast<const plqcd_hip_field *>(op.fuse_x)})
+         if (!f || f->lat != g->lat || f->prec != g->prec || f->nrhs != in->nrhs || f == in || f == out)
+            return plqcd::set_error(1, "hop: CG-update operands must be distinct fields like in");
+      if (op.fuse_p == op.fuse_x) return plqcd::set_error(1, "hop: CG-update p and x must be distinct");
+   }
    switch (g->prec * 100 + g->links) {
       case PLQCD_PREC_F64 * 100 + 18: return plqcd::hop_launch<PLQCD_PREC_F64, 18>(g, in, out, parity, mode, op);
       case PLQCD_PREC_F64 * 100 + 12: return plqcd::hop_launch<PLQCD_PREC_F64, 12>(g, in, out, parity, mode, op);
@@ -106,19 +126,35 @@ extern "C" int plqcd_hip_hop(const plqcd_hip_gauge *g, const plqcd_hip_field *in
    return plqcd::dslash(g, in, out, parity, mode, plqcd::DslashOp{});
 }
 
-extern "C" int plqcd_hip_hop_tuning(int prec, int order, int group, int block, int tmarch, int rb)
+extern "C" int plqcd_hip_hop_tuning(int prec, int order, int group, int block, int tmarch, int rb, int tile_y,
+                                    int tile_z)
 {
    if (prec < -1 || prec > PLQCD_PREC_F16 || order < 0 || order > 2 || group < 1 || group > 4096 ||
        (block != 64 && block != 128 && block != 256 && block != 512) || tmarch < 0 || tmarch > 4096 ||
-       (rb != 1 && rb != 2 && rb != 4))
+       (rb != 1 && rb != 2 && rb != 4) || tile_y < 0 || tile_z < 0 || tile_y > 4096 || tile_z > 4096 ||
+       ((tile_y == 0) != (tile_z == 0)))
       return plqcd::set_error(1, "plqcd_hip_hop_tuning: bad parameters (prec %d, order %d, group %d, block %d, "
-                                 "tmarch %d, rb %d)", prec, order, group, block, tmarch, rb);
+                                 "tmarch %d, rb %d, tile %d x %d)", prec, order, group, block, tmarch, rb, tile_y, tile_z);
    for (int p = 0; p < 3; p++)
-      if (prec < 0 || p == prec) g_tuning[p] = HopTuning{order, group, block, tmarch, rb};
+      if (prec < 0 || p == prec) {
+         g_tuning[p] = HopTuning{order, group, block, tmarch, rb, tile_y, tile_z};
+         g_tuning_set[p] = true;
+      }
    return 0;
 }
 
-extern "C" int plqcd_hip_hop_get_tuning(int prec, int out[5])
+extern "C" int plqcd_hip_hop_tuning_reset(int prec)
+{
+   if (prec < -1 || prec > PLQCD_PREC_F16) return plqcd::set_error(1, "plqcd_hip_hop_tuning_reset: bad precision");
+   for (int p = 0; p < 3; p++)
+      if (prec < 0 || p == prec) {
+         g_tuning[p] = kOneRhs[p];
+         g_tuning_set[p] = false;
+      }
+   return 0;
+}
+
+extern "C" int plqcd_hip_hop_get_tuning(int prec, int out[7])
 {
    if (prec < 0 || prec > PLQCD_PREC_F16 || !out) return plqcd::set_error(1, "plqcd_hip_hop_get_tuning: bad arguments");
    out[0] = g_tuning[prec].order;
@@ -126,5 +162,7 @@ extern "C" int plqcd_hip_hop_get_tuning(int prec, int out[5])
    out[2] = g_tuning[prec].block;
    out[3] = g_tuning[prec].tmarch;
    out[4] = g_tuning[prec].rb;
+   out[5] = g_tuning[prec].ty;
+   out[6] = g_tuning[prec].tz;
    return 0;
 }

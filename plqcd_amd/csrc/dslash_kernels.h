@@ -229,20 +229,53 @@ __device__ __forceinline__ int logical_block(const HopArgs &a)
    return b;
 }
 
+// Operands of the CG-update epilogue (EPI = 2): the search direction p, the
+// iterate x and the per-RHS step alpha.
+template <class SIO>
+struct CgFuse {
+   SIO p, x;
+   const double *alpha;
+};
+
 // Epilogue: EPI = 0 stores H in; EPI = 1 stores in(x) - kappa_r * (H in)(x).
+// EPI = 2 forms Ap = in(x) - kappa_r * (H in)(x) in registers and applies the
+// CG update at the site instead of storing it: x += alpha_r p, out -= alpha_r
+// Ap (out holds the residual), in the working precision of the kernel (float
+// for F32/F16 storage, double for F64).
 // With NORM the site's |out|^2 joins the deterministic chunk reduction.
 template <int EPI, bool NORM, class SIO>
 __device__ __forceinline__ double epilogue(const SIO &in, const SIO &out, const HopArgs &a, int r, int p, int j,
-                                           typename SIO::C acc[12], const typename SIO::C *cen)
+                                           typename SIO::C acc[12], const typename SIO::C *cen, const CgFuse<SIO> &f)
 {
    typedef typename SIO::real real;
    typedef typename SIO::C C;
-   if (EPI == 1) {
+   if (EPI >= 1) {
       C c0[12];
       if (cen == nullptr) in.load(p, j, c0);
       const real kap = (real)a.kappa[r];
 #pragma unroll
       for (int m = 0; m < 12; m++) acc[m] = cxmky(cen ? cen[m] : c0[m], kap, acc[m]);
+   }
+   if (EPI == 2) {
+      const real al = (real)f.alpha[r];
+      {
+         C ps[12], xs[12];
+         f.p.at_rhs(r).load(p, j, ps);
+         f.x.at_rhs(r).load(p, j, xs);
+#pragma unroll
+         for (int m = 0; m < 12; m++) xs[m] = C{fma(al, ps[m].x, xs[m].x), fma(al, ps[m].y, xs[m].y)};
+         f.x.at_rhs(r).store(p, j, xs);
+      }
+      C rs[12];
+      out.load(p, j, rs);
+      double nrm = 0.0;
+#pragma unroll
+      for (int m = 0; m < 12; m++) {
+         rs[m] = C{fma(-al, acc[m].x, rs[m].x), fma(-al, acc[m].y, rs[m].y)};
+         nrm = fma((double)rs[m].y, (double)rs[m].y, fma((double)rs[m].x, (double)rs[m].x, nrm));
+      }
+      out.store(p, j, rs);
+      return NORM ? nrm : 0.0;
    }
    double nrm = 0.0;
    if (NORM) {
@@ -255,7 +288,8 @@ __device__ __forceinline__ double epilogue(const SIO &in, const SIO &out, const 
 
 // One block of the gather kernel: logical block `blk` of the launch.
 template <int MODE, bool DAG, int EPI, bool NORM, int BLOCK, class SIO, class GIO>
-__device__ __forceinline__ void hop_block(const SIO &in, const SIO &out, const GIO &g, const HopArgs &a, int blk)
+__device__ __forceinline__ void hop_block(const SIO &in, const SIO &out, const GIO &g, const HopArgs &a, int blk,
+                                          const CgFuse<SIO> &f)
 {
    typedef typename SIO::C C;
    int p;
@@ -279,16 +313,16 @@ __device__ __forceinline__ void hop_block(const SIO &in, const SIO &out, const G
       hop_dir<1, MODE, DAG>(si, g, p, j, n, acc);
       hop_dir<2, MODE, DAG>(si, g, p, j, n, acc);
       hop_dir<3, MODE, DAG>(si, g, p, j, n, acc);
-      nrm = epilogue<EPI, NORM>(si, so, a, r, p, j, acc, (const C *)nullptr);
+      nrm = epilogue<EPI, NORM>(si, so, a, r, p, j, acc, (const C *)nullptr, f);
    }
    if (NORM) chunk_reduce<BLOCK>(nrm, js, r, a.R, p * a.nchunk + (blk * a.spb + js) / REDUCE_CHUNK,
                                  j - (js & (REDUCE_CHUNK - 1)) < a.Vh, a.partial + (size_t)r * 2 * a.nchunk);
 }
 
 template <int MODE, bool DAG, int EPI, bool NORM, int BLOCK, class SIO, class GIO>
-__global__ __launch_bounds__(BLOCK) void hop_kernel(SIO in, SIO out, GIO g, HopArgs a)
+__global__ __launch_bounds__(BLOCK) void hop_kernel(SIO in, SIO out, GIO g, HopArgs a, CgFuse<SIO> f)
 {
-   hop_block<MODE, DAG, EPI, NORM, BLOCK>(in, out, g, a, logical_block(a));
+   hop_block<MODE, DAG, EPI, NORM, BLOCK>(in, out, g, a, logical_block(a), f);
 }
 
 // RHS-blocked gather: each lane handles RB consecutive right-hand sides of
@@ -296,7 +330,8 @@ __global__ __launch_bounds__(BLOCK) void hop_kernel(SIO in, SIO out, GIO g, HopA
 // RB right-hand sides.  Lanes per site G = R / RB.  Every RHS goes through
 // the same operation sequence as in hop_block.
 template <int MODE, bool DAG, int EPI, bool NORM, int BLOCK, int RB, class SIO, class GIO>
-__device__ __forceinline__ void hop_block_rb(const SIO &in, const SIO &out, const GIO &g, const HopArgs &a, int blk)
+__device__ __forceinline__ void hop_block_rb(const SIO &in, const SIO &out, const GIO &g, const HopArgs &a, int blk,
+                                             const CgFuse<SIO> &f)
 {
    typedef typename SIO::C C;
    constexpr int SF = DAG ? 1 : -1, SB = -SF;
@@ -352,7 +387,7 @@ __device__ __forceinline__ void hop_block_rb(const SIO &in, const SIO &out, cons
 #pragma unroll
       for (int k = 0; k < RB; k++)
          nrm[k] = epilogue<EPI, NORM>(in.at_rhs(gi * RB + k), out.at_rhs(gi * RB + k), a, gi * RB + k, p, j, acc[k],
-                                      (const C *)nullptr);
+                                      (const C *)nullptr, f);
    }
    if (NORM) {
 #pragma unroll
@@ -363,9 +398,9 @@ __device__ __forceinline__ void hop_block_rb(const SIO &in, const SIO &out, cons
 }
 
 template <int MODE, bool DAG, int EPI, bool NORM, int BLOCK, int RB, class SIO, class GIO>
-__global__ __launch_bounds__(BLOCK) void hop_rb_kernel(SIO in, SIO out, GIO g, HopArgs a)
+__global__ __launch_bounds__(BLOCK) void hop_rb_kernel(SIO in, SIO out, GIO g, HopArgs a, CgFuse<SIO> f)
 {
-   hop_block_rb<MODE, DAG, EPI, NORM, BLOCK, RB>(in, out, g, a, logical_block(a));
+   hop_block_rb<MODE, DAG, EPI, NORM, BLOCK, RB>(in, out, g, a, logical_block(a), f);
 }
 
 // One application of H on each of several independent lattices (each with its
@@ -389,7 +424,7 @@ __global__ __launch_bounds__(BLOCK) void hop_batch_kernel(const BatchItem<SIO, G
       int it = 0;
       while (it + 1 < nitems && b >= items[it + 1].first) it++;
       const BatchItem<SIO, GIO> &I = items[it];
-      hop_block<MODE, false, 0, false, BLOCK>(I.in, I.out, I.g, I.a, b - I.first);
+      hop_block<MODE, false, 0, false, BLOCK>(I.in, I.out, I.g, I.a, b - I.first, CgFuse<SIO>{});
    }
 }
 
@@ -403,7 +438,7 @@ __global__ __launch_bounds__(BLOCK) void hop_batch_kernel(const BatchItem<SIO, G
 // the index of the site inside its parity advances by S = L0/2*L1*L2 per
 // t-slice.  Terms are accumulated in the same order as in hop_kernel.
 template <bool DAG, int EPI, bool NORM, int BLOCK, class SIO, class GIO>
-__global__ __launch_bounds__(BLOCK) void hop_tmarch_kernel(SIO in, SIO out, GIO g, HopArgs a)
+__global__ __launch_bounds__(BLOCK) void hop_tmarch_kernel(SIO in, SIO out, GIO g, HopArgs a, CgFuse<SIO> fuse)
 {
    typedef typename SIO::C C;
    constexpr int SF = DAG ? 1 : -1;  // forward projector sign
@@ -483,7 +518,7 @@ __global__ __launch_bounds__(BLOCK) void hop_tmarch_kernel(SIO in, SIO out, GIO 
             proj<3, -SF>(cen, kk);
             mul_udag(u, kk, b);
          }
-         nrm = epilogue<EPI, NORM>(si, so, a, r, p, j, acc, cen);
+         nrm = epilogue<EPI, NORM>(si, so, a, r, p, j, acc, cen, fuse);
 #pragma unroll
          for (int m = 0; m < 12; m++) cen[m] = f[m];
       }
@@ -492,31 +527,197 @@ __global__ __launch_bounds__(BLOCK) void hop_tmarch_kernel(SIO in, SIO out, GIO 
    }
 }
 
-// kernel variants: gather (1 RHS per lane), t-march, RHS-blocked gather
-enum Variant { V_GATHER = 0, V_MARCH = 1, V_RB2 = 2, V_RB4 = 4 };
+// One direction of the tiled kernel: neighbours inside the tile come from the
+// LDS slots su / sd (>= 0), the others from memory; same operation sequence
+// as hop_fwd + hop_bwd.
+template <int MU, bool DAG, int BLOCK, class SIO, class GIO>
+__device__ __forceinline__ void tile_dir(const SIO &in, const GIO &g, const typename SIO::C (*buf)[BLOCK], int p,
+                                         int j, int base, int cu, int cd, int su, int sd, typename SIO::C acc[12])
+{
+   typedef typename SIO::C C;
+   constexpr int SG = DAG ? 1 : -1;
+   C f[12], h[6], u[9], r[6];
+   if (su >= 0) {
+#pragma unroll
+      for (int m = 0; m < 12; m++) f[m] = buf[m][su];
+   } else {
+      in.load(1 - p, cu + base, f);
+   }
+   proj<MU, SG>(f, h);
+   g.load(p, MU, j, u);
+   mul_u(u, h, r);
+   recon<MU, SG, true>(acc, r);
+   if (sd >= 0) {
+#pragma unroll
+      for (int m = 0; m < 12; m++) f[m] = buf[m][sd];
+   } else {
+      in.load(1 - p, cd + base, f);
+   }
+   proj<MU, -SG>(f, h);
+   g.load(1 - p, MU, cd + base, u);
+   mul_udag(u, h, r);
+   recon<MU, -SG, true>(acc, r);
+}
+
+// LDS-tiled t-march (wilson mode, full lattice, one RHS).  A block owns the
+// columns of a tile of Lxh x ty x tz (x, y, z) sites -- whole x rows -- for
+// both parities: lanes q = 0 and 1 of a column hold its two x-adjacent sites,
+// and the block walks T t-slices.  The centre spinors of a t-slice are in LDS
+// (two buffers, one barrier per slice), and the x, y and z
+// neighbours inside the tile are read from there; only the tile's y/z halo and
+// the t+1 spinor are loaded from memory.  The centre spinor and the backward
+// t term stay in LDS rather than in registers.  Links and the t direction are
+// handled as in
+// hop_tmarch_kernel, and terms are accumulated in the order of hop_kernel.
+// BLOCK = 2 Lxh ty tz lanes.
+template <bool DAG, int EPI, bool NORM, int BLOCK, class SIO, class GIO>
+__global__ __launch_bounds__(BLOCK) void hop_tile_kernel(SIO in, SIO out, GIO g, HopArgs a, CgFuse<SIO> fuse)
+{
+   typedef typename SIO::C C;
+   constexpr int SF = DAG ? 1 : -1;  // forward projector sign
+   __shared__ C sm[2][12][BLOCK];
+   __shared__ C sb[6][BLOCK];  // the lane's backward t term b (read and written by that lane only)
+   const int S = a.S;
+   constexpr int ncol = BLOCK / 2;
+   const int nty = a.L[1] / a.ty;
+   const int ntile = nty * (a.L[2] / a.tz);
+   const int lb = logical_block(a);
+   const int tile = lb % ntile;
+   const int chunk = lb / ntile;
+   const int tid = threadIdx.x;
+   const int q = tid >= ncol ? 1 : 0;  // parity of the lane's first site
+   const int lc = tid - q * ncol;      // xh + Lxh * (ly + ty * lz)
+   const int xh = lc % a.Lxh;
+   const int lyz = lc / a.Lxh;
+   const int ly = lyz % a.ty, lz = lyz / a.ty;
+   const int y0 = (tile % nty) * a.ty, z0 = (tile / nty) * a.tz;
+   const int y = y0 + ly, z = z0 + lz;
+   const int row = y + a.L[1] * z;
+   const int c = row * a.Lxh + xh;
+   const int t0 = chunk * a.T;
+   const int x = 2 * xh + ((q + y + z + t0) & 1);  // the same at every t-slice
+   int cu[3], cd[3], su[3], sd[3];
+   {
+      const int oth = (1 - q) * ncol;  // lanes of the other parity
+      const int xp = (x + 1 == a.L[0]) ? 0 : x + 1;
+      const int xm = (x == 0) ? a.L[0] - 1 : x - 1;
+      cu[0] = row * a.Lxh + (xp >> 1);
+      cd[0] = row * a.Lxh + (xm >> 1);
+      su[0] = oth + lc - xh + (xp >> 1);
+      sd[0] = oth + lc - xh + (xm >> 1);
+      const int yp = (y + 1 == a.L[1]) ? 0 : y + 1;
+      const int ym = (y == 0) ? a.L[1] - 1 : y - 1;
+      const int zp = (z + 1 == a.L[2]) ? 0 : z + 1;
+      const int zm = (z == 0) ? a.L[2] - 1 : z - 1;
+      cu[1] = (yp + a.L[1] * z) * a.Lxh + xh;
+      cd[1] = (ym + a.L[1] * z) * a.Lxh + xh;
+      cu[2] = (y + a.L[1] * zp) * a.Lxh + xh;
+      cd[2] = (y + a.L[1] * zm) * a.Lxh + xh;
+      const int lyp = yp - y0, lym = ym - y0, lzp = zp - z0, lzm = zm - z0;
+      su[1] = (lyp >= 0 && lyp < a.ty) ? oth + xh + a.Lxh * (lyp + a.ty * lz) : -1;
+      sd[1] = (lym >= 0 && lym < a.ty) ? oth + xh + a.Lxh * (lym + a.ty * lz) : -1;
+      su[2] = (lzp >= 0 && lzp < a.tz) ? oth + xh + a.Lxh * (ly + a.ty * lzp) : -1;
+      sd[2] = (lzm >= 0 && lzm < a.tz) ? oth + xh + a.Lxh * (ly + a.ty * lzm) : -1;
+   }
+   const int Lt = a.L[3];
+   // prologue: psi(t0) into the first buffer; b = U_t(t0-1)^+ (1 -+ g_t) psi(t0-1)
+   {
+      C f[12], u[9], kk[6], b[6];
+      in.load(q, c + t0 * S, f);
+#pragma unroll
+      for (int m = 0; m < 12; m++) sm[0][m][tid] = f[m];
+      const int tm = (t0 == 0 ? Lt : t0) - 1;
+      in.load(q ^ 1, c + tm * S, f);
+      proj<3, -SF>(f, kk);
+      g.load(q ^ 1, 3, c + tm * S, u);
+      mul_udag(u, kk, b);
+#pragma unroll
+      for (int m = 0; m < 6; m++) sb[m][tid] = b[m];
+   }
+   for (int s = 0; s < a.T; s++) {
+      const int t = t0 + s;
+      const int p = q ^ (s & 1);
+      const int base = t * S;
+      const int j = c + base;
+      // buf holds the centre spinors of this t-slice; the t+1 spinors go to
+      // nxt, whose readers (the previous slice) have all passed this barrier
+      C(*buf)[BLOCK] = sm[s & 1];
+      C(*nxt)[BLOCK] = sm[(s + 1) & 1];
+      __syncthreads();
+      C acc[12];
+#pragma unroll
+      for (int m = 0; m < 12; m++) acc[m] = C{0, 0};
+      // the scheduling barriers keep each direction's loads in its own
+      // section, which holds the kernel within 256 VGPRs without spills
+      tile_dir<0, DAG, BLOCK>(in, g, buf, p, j, base, cu[0], cd[0], su[0], sd[0], acc);
+      __builtin_amdgcn_sched_barrier(0);
+      tile_dir<1, DAG, BLOCK>(in, g, buf, p, j, base, cu[1], cd[1], su[1], sd[1], acc);
+      __builtin_amdgcn_sched_barrier(0);
+      tile_dir<2, DAG, BLOCK>(in, g, buf, p, j, base, cu[2], cd[2], su[2], sd[2], acc);
+      __builtin_amdgcn_sched_barrier(0);
+      {
+         C f[12], h[6], u[9], rr[6], kk[6], b[6];
+         const int tp = (t + 1 == Lt) ? 0 : t + 1;
+         in.load(p ^ 1, c + tp * S, f);
+#pragma unroll
+         for (int m = 0; m < 12; m++) nxt[m][tid] = f[m];
+         proj<3, SF>(f, h);
+         g.load(p, 3, j, u);
+         mul_u(u, h, rr);
+         recon<3, SF, true>(acc, rr);
+#pragma unroll
+         for (int m = 0; m < 6; m++) b[m] = sb[m][tid];
+         recon<3, -SF, true>(acc, b);
+#pragma unroll
+         for (int m = 0; m < 12; m++) f[m] = buf[m][tid];
+         proj<3, -SF>(f, kk);
+         mul_udag(u, kk, b);
+#pragma unroll
+         for (int m = 0; m < 6; m++) sb[m][tid] = b[m];
+      }
+      double nrm;
+      if (EPI >= 1) {
+         C cen[12];
+#pragma unroll
+         for (int m = 0; m < 12; m++) cen[m] = buf[m][tid];
+         nrm = epilogue<EPI, NORM>(in, out, a, 0, p, j, acc, cen, fuse);
+      } else {
+         nrm = epilogue<EPI, NORM>(in, out, a, 0, p, j, acc, (const C *)nullptr, fuse);
+      }
+      if (NORM) chunk_reduce<BLOCK>(nrm, lc, 0, 1, p * a.nchunk + j / REDUCE_CHUNK, true, a.partial);
+   }
+}
+
+// kernel variants: gather (1 RHS per lane), t-march, RHS-blocked gather, LDS tile
+enum Variant { V_GATHER = 0, V_MARCH = 1, V_RB2 = 2, V_RB4 = 4, V_TILE = 8 };
 
 template <int MODE, bool DAG, int EPI, bool NORM, int BLOCK, class S, class G>
-void launch_one(int variant, dim3 grid, dim3 block, hipStream_t st, S si, S so, G gg, const HopArgs &a)
+void launch_one(int variant, dim3 grid, dim3 block, hipStream_t st, S si, S so, G gg, const HopArgs &a,
+                const CgFuse<S> &f)
 {
    if (variant == V_MARCH) {
       if constexpr (MODE == PLQCD_HOP_WILSON)
-         hipLaunchKernelGGL((hop_tmarch_kernel<DAG, EPI, NORM, BLOCK, S, G>), grid, block, 0, st, si, so, gg, a);
+         hipLaunchKernelGGL((hop_tmarch_kernel<DAG, EPI, NORM, BLOCK, S, G>), grid, block, 0, st, si, so, gg, a, f);
+   } else if (variant == V_TILE) {
+      // two LDS buffers of BLOCK spinors and BLOCK half-spinors must fit the CU's 160 KiB
+      if constexpr (MODE == PLQCD_HOP_WILSON && (2 * 12 + 6) * BLOCK * sizeof(typename S::C) <= 136 * 1024)
+         hipLaunchKernelGGL((hop_tile_kernel<DAG, EPI, NORM, BLOCK, S, G>), grid, block, 0, st, si, so, gg, a, f);
    } else if (variant == V_RB2) {
-      hipLaunchKernelGGL((hop_rb_kernel<MODE, DAG, EPI, NORM, BLOCK, 2, S, G>), grid, block, 0, st, si, so, gg, a);
+      hipLaunchKernelGGL((hop_rb_kernel<MODE, DAG, EPI, NORM, BLOCK, 2, S, G>), grid, block, 0, st, si, so, gg, a, f);
    } else if (variant == V_RB4) {
-      hipLaunchKernelGGL((hop_rb_kernel<MODE, DAG, EPI, NORM, BLOCK, 4, S, G>), grid, block, 0, st, si, so, gg, a);
+      hipLaunchKernelGGL((hop_rb_kernel<MODE, DAG, EPI, NORM, BLOCK, 4, S, G>), grid, block, 0, st, si, so, gg, a, f);
    } else {
-      hipLaunchKernelGGL((hop_kernel<MODE, DAG, EPI, NORM, BLOCK, S, G>), grid, block, 0, st, si, so, gg, a);
+      hipLaunchKernelGGL((hop_kernel<MODE, DAG, EPI, NORM, BLOCK, S, G>), grid, block, 0, st, si, so, gg, a, f);
    }
 }
 
 template <int MODE, bool DAG, int EPI, bool NORM, class S, class G>
 void launch_block(int maxblock, int variant, dim3 grid, dim3 block, hipStream_t st, S si, S so, G gg,
-                  const HopArgs &a)
+                  const HopArgs &a, const CgFuse<S> &f = CgFuse<S>{})
 {
-   if (maxblock <= 128) launch_one<MODE, DAG, EPI, NORM, 128>(variant, grid, block, st, si, so, gg, a);
-   else if (maxblock <= 256) launch_one<MODE, DAG, EPI, NORM, 256>(variant, grid, block, st, si, so, gg, a);
-   else launch_one<MODE, DAG, EPI, NORM, 512>(variant, grid, block, st, si, so, gg, a);
+   if (maxblock <= 128) launch_one<MODE, DAG, EPI, NORM, 128>(variant, grid, block, st, si, so, gg, a, f);
+   else if (maxblock <= 256) launch_one<MODE, DAG, EPI, NORM, 256>(variant, grid, block, st, si, so, gg, a, f);
+   else launch_one<MODE, DAG, EPI, NORM, 512>(variant, grid, block, st, si, so, gg, a, f);
 }
 
 template <int PREC, int LINKS>
@@ -530,7 +731,7 @@ int launch_t(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip_fiel
    const S si = spinor_io<PREC>(in), so = spinor_io<PREC>(out);
    const G gg = gauge_io<PREC, LINKS>(g);
    const int R = in->nrhs;
-   HopTuning tu = g_tuning[PREC];
+   HopTuning tu = plqcd::default_tuning(PREC, R);
    plqcd::lookup_tuning(plqcd::TuneKey{{lat->L[0], lat->L[1], lat->L[2], lat->L[3]}, PREC, LINKS, in->nrhs}, &tu);
    // RHS per lane (RHS-blocked gather) when it divides R
    const int rb = (tu.rb == 2 || tu.rb == 4) && R % tu.rb == 0 ? tu.rb : 1;
@@ -551,10 +752,18 @@ int launch_t(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip_fiel
    a.partial = op.partial;
    a.nchunk = plqcd::reduce_chunks(lat->Vh);
    const bool dag = op.dag != 0;
-   const int epi = op.kappa ? 1 : 0;
+   const int epi = op.alpha ? 2 : op.kappa ? 1 : 0;
    const bool norm = op.partial != nullptr;
    if ((dag || epi || norm) && mode != PLQCD_HOP_WILSON)
       return plqcd::set_error(2, "hop: M, M^+ and norms need the wilson mode");
+   if (epi == 2 && (!dag || !norm || !op.kappa || !op.fuse_p || !op.fuse_x))
+      return plqcd::set_error(2, "hop: the CG-update epilogue needs M^+, a norm, kappa, p and x");
+   CgFuse<S> fu{};
+   if (epi == 2) {
+      fu.p = spinor_io<PREC>(op.fuse_p);
+      fu.x = spinor_io<PREC>(op.fuse_x);
+      fu.alpha = op.alpha;
+   }
    if (norm && a.nparity != 2) return plqcd::set_error(2, "hop: norms are only fused into full-lattice launches");
    const int T = tu.tmarch;
    // the t-march kernel serves full-lattice wilson applications whose time
@@ -562,26 +771,43 @@ int launch_t(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip_fiel
    // reduction chunks)
    const bool march = rb == 1 && T > 0 && mode == PLQCD_HOP_WILSON && a.nparity == 2 && lat->L[3] % T == 0 &&
                       (!norm || a.S % REDUCE_CHUNK == 0);
-   const int variant = march ? V_MARCH : (rb == 2 ? V_RB2 : rb == 4 ? V_RB4 : V_GATHER);
+   // the LDS-tiled t-march: one RHS, a tile of whole x rows dividing the y and
+   // z extents, 2 Lxh ty tz lanes per block (at most 256 with F64 storage for
+   // the LDS), and, with a norm, reduction chunks that do not leave a tile row
+   const int tlanes = 2 * lat->Lxh * tu.ty * tu.tz;
+   const bool tiled = march && R == 1 && tu.ty > 0 && tu.tz > 0 && lat->L[1] % tu.ty == 0 &&
+                      lat->L[2] % tu.tz == 0 &&
+                      (tlanes == 128 || tlanes == 256 || (tlanes == 512 && PREC != PLQCD_PREC_F64)) &&
+                      (!norm || (lat->Lxh * tu.ty) % REDUCE_CHUNK == 0);
+   const int variant = tiled ? V_TILE : march ? V_MARCH : (rb == 2 ? V_RB2 : rb == 4 ? V_RB4 : V_GATHER);
    a.T = march ? T : 1;
-   const int nblk = march ? ((a.S + a.spb - 1) / a.spb) * (lat->L[3] / T) : (lat->Vh + a.spb - 1) / a.spb;
-   const int nb = nblk * a.nparity;
+   int nb, nthreads = threads, mblock = maxblock;
+   if (tiled) {
+      a.ty = tu.ty;
+      a.tz = tu.tz;
+      nb = (lat->L[1] / tu.ty) * (lat->L[2] / tu.tz) * (lat->L[3] / T);
+      nthreads = mblock = tlanes;
+   } else {
+      const int nblk = march ? ((a.S + a.spb - 1) / a.spb) * (lat->L[3] / T) : (lat->Vh + a.spb - 1) / a.spb;
+      nb = nblk * a.nparity;
+   }
    // the remapped orders need whole rounds of 8 XCDs (and of 8*group blocks)
    a.order = tu.order;
    a.group = tu.group > 0 ? tu.group : 1;
    if ((a.order == 1 && nb % 8) || (a.order == 2 && nb % (8 * a.group))) a.order = 0;
-   const dim3 grid(nb), block(threads);
+   const dim3 grid(nb), block(nthreads);
    hipStream_t st = lat->stream;
    if (mode == PLQCD_HOP_COMPAT) {
-      launch_block<PLQCD_HOP_COMPAT, false, 0, false>(maxblock, variant, grid, block, st, si, so, gg, a);
+      launch_block<PLQCD_HOP_COMPAT, false, 0, false>(mblock, variant, grid, block, st, si, so, gg, a);
    } else if (!dag) {
-      if (epi == 0) launch_block<PLQCD_HOP_WILSON, false, 0, false>(maxblock, variant, grid, block, st, si, so, gg, a);
-      else if (!norm) launch_block<PLQCD_HOP_WILSON, false, 1, false>(maxblock, variant, grid, block, st, si, so, gg, a);
-      else launch_block<PLQCD_HOP_WILSON, false, 1, true>(maxblock, variant, grid, block, st, si, so, gg, a);
+      if (epi == 0) launch_block<PLQCD_HOP_WILSON, false, 0, false>(mblock, variant, grid, block, st, si, so, gg, a);
+      else if (!norm) launch_block<PLQCD_HOP_WILSON, false, 1, false>(mblock, variant, grid, block, st, si, so, gg, a);
+      else launch_block<PLQCD_HOP_WILSON, false, 1, true>(mblock, variant, grid, block, st, si, so, gg, a);
    } else {
-      if (epi == 0) launch_block<PLQCD_HOP_WILSON, true, 0, false>(maxblock, variant, grid, block, st, si, so, gg, a);
-      else if (!norm) launch_block<PLQCD_HOP_WILSON, true, 1, false>(maxblock, variant, grid, block, st, si, so, gg, a);
-      else launch_block<PLQCD_HOP_WILSON, true, 1, true>(maxblock, variant, grid, block, st, si, so, gg, a);
+      if (epi == 0) launch_block<PLQCD_HOP_WILSON, true, 0, false>(mblock, variant, grid, block, st, si, so, gg, a);
+      else if (epi == 2) launch_block<PLQCD_HOP_WILSON, true, 2, true>(mblock, variant, grid, block, st, si, so, gg, a, fu);
+      else if (!norm) launch_block<PLQCD_HOP_WILSON, true, 1, false>(mblock, variant, grid, block, st, si, so, gg, a);
+      else launch_block<PLQCD_HOP_WILSON, true, 1, true>(mblock, variant, grid, block, st, si, so, gg, a);
    }
    PLQCD_HIP_TRY(hipGetLastError());
    return 0;

@@ -168,6 +168,36 @@ struct SpinorIO<PLQCD_PREC_F16> {
 };
 
 // Accessor of a whole field (RHS 0; bind a lane with at_rhs).
+// A site's spinor of any storage precision converted to / from double.
+template <int P>
+__device__ __forceinline__ void load_d(const SpinorIO<P> &f, int par, int j, d2 s[12])
+{
+   typename SpinorIO<P>::C t[12];
+   f.load(par, j, t);
+#pragma unroll
+   for (int c = 0; c < 12; c++) s[c] = d2{(double)t[c].x, (double)t[c].y};
+}
+
+template <int P>
+__device__ __forceinline__ void store_d(const SpinorIO<P> &f, int par, int j, const d2 s[12])
+{
+   typedef typename SpinorIO<P>::C C;
+   typedef typename SpinorIO<P>::real real;
+   C t[12];
+#pragma unroll
+   for (int c = 0; c < 12; c++) t[c] = C{(real)s[c].x, (real)s[c].y};
+   f.store(par, j, t);
+}
+
+// |s|^2 of a site in a fixed order
+__device__ __forceinline__ double norm2(const d2 s[12])
+{
+   double n = 0.0;
+#pragma unroll
+   for (int c = 0; c < 12; c++) n = fma(s[c].y, s[c].y, fma(s[c].x, s[c].x, n));
+   return n;
+}
+
 template <int PREC>
 inline SpinorIO<PREC> spinor_io(const plqcd_hip_field *f)
 {

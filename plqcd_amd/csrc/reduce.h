@@ -5,7 +5,8 @@
 //   1. each chunk of REDUCE_CHUNK consecutive sites (within one parity) is
 //      summed in a fixed binary tree, giving partial[r][chunk]
 //      (chunk = parity * nchunk + j / REDUCE_CHUNK);
-//   2. reduce_partials() sums the chunks of each RHS in a fixed order.
+//   2. reduce_partials() sums the chunks of each RHS in an order fixed by the
+//      chunk count (segments of 4096 chunks, then the segment sums).
 // A right-hand side therefore gets bitwise the same sums alone or batched.
 #pragma once
 
@@ -17,7 +18,8 @@ namespace plqcd {
 inline int reduce_chunks(int Vh) { return (Vh + REDUCE_CHUNK - 1) / REDUCE_CHUNK; }
 
 // out[r] = sum of partial[r][first .. first+n) for r < R (fixed order), on `st`.
-int reduce_partials(const double *partial, size_t stride, int first, int n, int R, double *out, hipStream_t st);
+// The partials are overwritten.
+int reduce_partials(double *partial, size_t stride, int first, int n, int R, double *out, hipStream_t st);
 }  // namespace plqcd
 
 // Stage 1 inside a block whose lanes are (js, r), tid = js * R + r, with

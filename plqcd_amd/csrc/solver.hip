@@ -12,7 +12,8 @@
 // its tolerance.
 //
 // One iteration is a fixed sequence of kernels (beta, p update, M p with
-// |Mp|^2, M^+, alpha, x/r update with |r|^2) whose scalars live on the
+// |Mp|^2, alpha, then M^+ (Mp) with the x/r update and |r|^2 fused into its
+// epilogue, so Ap is never stored) whose scalars live on the
 // device; it is captured once into a hipGraph and replayed.  Only |r|^2 comes
 // back to the host each iteration, to decide reliable updates.
 #include <algorithm>
@@ -60,8 +61,8 @@ size_t CgEngine::bytes_needed(const plqcd_hip_lattice *lat, int sloppy, int S)
 {
    const size_t f64 = 2 * spinor_parity_bytes(PLQCD_PREC_F64, lat->Vh) * S;
    const size_t fs = 2 * spinor_parity_bytes(sloppy, lat->Vh) * S;
-   const size_t nf64 = sloppy == PLQCD_PREC_F64 ? 2 : 4;  // b, x (+ t1, t2)
-   return nf64 * f64 + 5 * fs + (size_t)S * 2 * reduce_chunks(lat->Vh) * sizeof(double) + (1 << 20);
+   const size_t nf64 = sloppy == PLQCD_PREC_F64 ? 3 : 4;  // b, x, t2 (+ t1)
+   return nf64 * f64 + 4 * fs + (size_t)S * 2 * reduce_chunks(lat->Vh) * sizeof(double) + (1 << 20);
 }
 
 int CgEngine::create(const plqcd_hip_gauge *g, const plqcd_hip_gauge *gs, int S, double delta, bool use_graph,
@@ -84,7 +85,7 @@ int CgEngine::create(const plqcd_hip_gauge *g, const plqcd_hip_gauge *gs, int S,
    e->S_ = S;
    e->use_graph_ = use_graph;
    const bool mixed = gs != g;
-   e->delta_ = delta > 0 ? delta : (!mixed ? 0.0 : (gs->prec == PLQCD_PREC_F16 ? 0.1 : 1e-3));
+   e->delta_ = delta > 0 ? delta : (!mixed ? 0.0 : (gs->prec == PLQCD_PREC_F16 ? 1e-2 : 1e-3));
    e->b_ = b;
    e->x_ = x;
    e->own_bx_ = b == nullptr;
@@ -105,15 +106,14 @@ int CgEngine::alloc(int sloppy)
       if (int rc = plqcd_hip_field_create(lat_, PLQCD_PREC_F64, S_, &b_)) return rc;
       if (int rc = plqcd_hip_field_create(lat_, PLQCD_PREC_F64, S_, &x_)) return rc;
    }
-   for (plqcd_hip_field **f : {&rs_, &ps_, &qs_, &aps_, &xs_})
+   for (plqcd_hip_field **f : {&rs_, &ps_, &qs_, &xs_})
       if (int rc = plqcd_hip_field_create(lat_, sloppy, S_, f)) return rc;
-   if (sloppy == PLQCD_PREC_F64) {  // the sloppy temporaries are free during a reliable update
-      t1_ = qs_;
-      t2_ = aps_;
+   if (sloppy == PLQCD_PREC_F64) {
+      t1_ = qs_;  // free during a reliable update
    } else {
       if (int rc = plqcd_hip_field_create(lat_, PLQCD_PREC_F64, S_, &t1_)) return rc;
-      if (int rc = plqcd_hip_field_create(lat_, PLQCD_PREC_F64, S_, &t2_)) return rc;
    }
+   if (int rc = plqcd_hip_field_create(lat_, PLQCD_PREC_F64, S_, &t2_)) return rc;
    PLQCD_HIP_TRY(hipMalloc(&partial_, partial_doubles(b_) * sizeof(double)));
    PLQCD_HIP_TRY(hipMalloc(&scal_, 7 * S_ * sizeof(double)));
    PLQCD_HIP_TRY(hipMalloc(&imask_, 2 * S_ * sizeof(int)));
@@ -136,11 +136,9 @@ CgEngine::~CgEngine()
    if (graph_) (void)hipGraphExecDestroy(graph_);
    if (graph_def_) (void)hipGraphDestroy(graph_def_);
    const bool alias = (t1_ == qs_);
-   for (plqcd_hip_field *f : {rs_, ps_, qs_, aps_, xs_}) plqcd_hip_field_destroy(f);
-   if (!alias) {
-      plqcd_hip_field_destroy(t1_);
-      plqcd_hip_field_destroy(t2_);
-   }
+   for (plqcd_hip_field *f : {rs_, ps_, qs_, xs_}) plqcd_hip_field_destroy(f);
+   if (!alias) plqcd_hip_field_destroy(t1_);
+   plqcd_hip_field_destroy(t2_);
    if (own_bx_) {
       plqcd_hip_field_destroy(b_);
       plqcd_hip_field_destroy(x_);
@@ -230,12 +228,16 @@ int CgEngine::iteration()
    m.partial = partial_;
    if (int rc = dslash(gs_, ps_, qs_, PLQCD_PARITY_ALL, PLQCD_HOP_WILSON, m)) return rc;
    if (int rc = reduce_partials(partial_, 2 * nc, 0, 2 * nc, S_, pap_d(), st)) return rc;
+   hipLaunchKernelGGL(k_alpha, dim3(1), dim3(kMaxSlots), 0, st, rr_d(), pap_d(), active_d(), alpha_d(), S_);
+   // Ap = M^+ q, fused with xs += alpha p, rs -= alpha Ap and |rs|^2
    DslashOp md;
    md.kappa = kappa_d();
    md.dag = 1;
-   if (int rc = dslash(gs_, qs_, aps_, PLQCD_PARITY_ALL, PLQCD_HOP_WILSON, md)) return rc;
-   hipLaunchKernelGGL(k_alpha, dim3(1), dim3(kMaxSlots), 0, st, rr_d(), pap_d(), active_d(), alpha_d(), S_);
-   if (int rc = blas_cg_update(xs_, ps_, rs_, aps_, alpha_d(), partial_)) return rc;
+   md.partial = partial_;
+   md.alpha = alpha_d();
+   md.fuse_p = ps_;
+   md.fuse_x = xs_;
+   if (int rc = dslash(gs_, qs_, rs_, PLQCD_PARITY_ALL, PLQCD_HOP_WILSON, md)) return rc;
    if (int rc = reduce_partials(partial_, 2 * nc, 0, 2 * nc, S_, rr_new_d(), st)) return rc;
    PLQCD_HIP_TRY(hipMemcpyAsync(host_, rr_new_d(), S_ * sizeof(double), hipMemcpyDeviceToHost, st));
    PLQCD_HIP_TRY(hipGetLastError());

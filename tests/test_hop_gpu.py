@@ -229,22 +229,19 @@ def test_benchmark_shape_32c64_f32_error(hp, orc):
 
 # ---------------------------------------------------------- launch shapes
 
-TUNINGS = [(0, 1, 256, 0, 1), (1, 1, 128, 0, 1), (2, 4, 64, 0, 1), (2, 16, 512, 0, 1),
-           (2, 16, 256, 2, 1), (1, 1, 128, 4, 1), (2, 8, 64, 8, 1), (0, 1, 256, 16, 1)]
+TUNINGS = [(0, 1, 256, 0, 1, 0, 0), (1, 1, 128, 0, 1, 0, 0), (2, 4, 64, 0, 1, 0, 0), (2, 16, 512, 0, 1, 0, 0),
+           (2, 16, 256, 2, 1, 0, 0), (1, 1, 128, 4, 1, 0, 0), (2, 8, 64, 8, 1, 0, 0), (0, 1, 256, 16, 1, 0, 0)]
 
 
 @pytest.mark.parametrize("tuning", TUNINGS)
 @pytest.mark.parametrize("prec,links,tol", [(0, 18, TOL64), (1, 12, TOL32), (2, 12, TOL16)])
 def test_launch_shapes_match_oracle(hp, orc, tuning, prec, links, tol):
     """Every block order, block size and t-march length gives the operator."""
-    import ctypes as C
     from plqcd_amd import capi
     L = (8, 4, 6, 16)
     q, u = inputs(hp, L)
     ref = orc.hopping_full(q, u, L, orc.MODE_WILSON)
     lib = capi.lib()
-    saved = (C.c_int * 5)()
-    lib.plqcd_hip_hop_get_tuning(prec, saved)
     assert lib.plqcd_hip_hop_tuning(prec, *tuning) == 0
     try:
         for mode in (1, 0):
@@ -252,7 +249,7 @@ def test_launch_shapes_match_oracle(hp, orc, tuning, prec, links, tol):
             r = ref if mode == 1 else orc.hopping_full(q, u, L, orc.MODE_COMPAT)
             assert site_rel_err(out, r) < tol, (tuning, mode)
     finally:
-        lib.plqcd_hip_hop_tuning(prec, *saved)
+        assert lib.plqcd_hip_hop_tuning_reset(prec) == 0
 
 
 # ------------------------------------------------- RHS-blocked gather kernel
@@ -260,14 +257,6 @@ def test_launch_shapes_match_oracle(hp, orc, tuning, prec, links, tol):
 def _set_tuning(prec, t):
     from plqcd_amd import capi
     assert capi.lib().plqcd_hip_hop_tuning(prec, *t) == 0
-
-
-def _get_tuning(prec):
-    import ctypes as C
-    from plqcd_amd import capi
-    saved = (C.c_int * 5)()
-    assert capi.lib().plqcd_hip_hop_get_tuning(prec, saved) == 0
-    return tuple(saved)
 
 
 @pytest.mark.parametrize("R", [4, 12])
@@ -286,13 +275,12 @@ def test_rhs_blocked_bitwise_equal_single_rhs(hp, prec, links, R):
     f1i, f1o = hp.Field(lat, prec), hp.Field(lat, prec)
     for r, q in enumerate(qs):
         fi.upload(q, r)
-    saved = _get_tuning(prec)
     try:
         for mode in (hp.HOP_WILSON, hp.HOP_COMPAT):
             for parity in (hp.PARITY_ALL, hp.PARITY_EVEN, hp.PARITY_ODD):
                 singles = []
                 for tm in (0, 4):
-                    _set_tuning(prec, (1, 1, 256, tm, 1))
+                    _set_tuning(prec, (1, 1, 256, tm, 1, 0, 0))
                     outs = []
                     for q in qs:
                         f1i.upload(q)
@@ -303,14 +291,63 @@ def test_rhs_blocked_bitwise_equal_single_rhs(hp, prec, links, R):
                 for r in range(R):
                     assert np.array_equal(singles[0][r], singles[1][r]), ("t-march", mode, parity, r)
                 for rb, blk in ((2, 128), (4, 256), (4, 64)):
-                    _set_tuning(prec, (2, 4, blk, 0, rb))
+                    _set_tuning(prec, (2, 4, blk, 0, rb, 0, 0))
                     for r in range(R):
                         fo.upload(np.zeros_like(qs[0]), r)
                     hp.hop(g, fi, fo, parity, mode)
                     for r in range(R):
                         assert np.array_equal(fo.download(r), singles[0][r]), (rb, blk, mode, parity, r)
     finally:
-        _set_tuning(prec, saved)
+        from plqcd_amd import capi
+        assert capi.lib().plqcd_hip_hop_tuning_reset(prec) == 0
         for f in (fi, fo, f1i, f1o):
             f.close()
         g.close(); lat.close()
+
+
+# ---------------------------------------------------- LDS-tiled t-march kernel
+
+TILES = [((8, 8, 8, 16), [(4, 4), (8, 2), (2, 8), (8, 4), (4, 8)]),
+         ((16, 16, 8, 8), [(8, 1), (4, 2), (1, 8), (16, 1), (4, 4)])]
+
+
+@pytest.mark.parametrize("L,tiles", TILES)
+@pytest.mark.parametrize("prec,links,tol", [(0, 18, TOL64), (1, 12, TOL32), (2, 12, TOL16)])
+def test_tile_kernel_bitwise_equal_tmarch(hp, orc, L, tiles, prec, links, tol):
+    """The LDS-tiled t-march gives bitwise the t-march's and the gather
+    kernel's output for every tile (including tiles spanning a whole extent,
+    whose periodic neighbours lie inside the tile), and matches the oracle."""
+    from plqcd_amd import capi
+    q, u = inputs(hp, L, 61, 62)
+    ref = orc.hopping_full(q, u, L, orc.MODE_WILSON)
+    try:
+        _set_tuning(prec, (1, 1, 256, 0, 1, 0, 0))
+        gather = device_hop(hp, L, q, u, hp.HOP_WILSON, prec=prec, links=links)
+        assert site_rel_err(gather, ref) < tol
+        for T in (4, 8):
+            _set_tuning(prec, (1, 1, 256, T, 1, 0, 0))
+            assert np.array_equal(device_hop(hp, L, q, u, hp.HOP_WILSON, prec=prec, links=links), gather), T
+            for ty, tz in tiles:
+                for order in (0, 1):
+                    _set_tuning(prec, (order, 1, 256, T, 1, ty, tz))
+                    out = device_hop(hp, L, q, u, hp.HOP_WILSON, prec=prec, links=links)
+                    assert np.array_equal(out, gather), (T, ty, tz, order)
+        # the half-lattice and compat launches ignore the tile
+        _set_tuning(prec, (1, 1, 256, 4, 1, 4, 4))
+        for parity in (hp.PARITY_EVEN, hp.PARITY_ODD):
+            half = device_hop(hp, L, q, u, hp.HOP_WILSON, parity=parity, prec=prec, links=links)
+            sl = slice(0, half.shape[0] // 2) if parity == hp.PARITY_EVEN else slice(half.shape[0] // 2, None)
+            assert np.array_equal(half[sl], gather[sl])
+        comp = device_hop(hp, L, q, u, hp.HOP_COMPAT, prec=prec, links=links)
+        assert site_rel_err(comp, orc.hopping_full(q, u, L, orc.MODE_COMPAT)) < tol
+    finally:
+        assert capi.lib().plqcd_hip_hop_tuning_reset(prec) == 0
+
+
+def test_tuning_rejects_bad_tiles(hp):
+    from plqcd_amd import capi
+    lib = capi.lib()
+    assert lib.plqcd_hip_hop_tuning(0, 1, 1, 256, 4, 1, 4, 0) != 0
+    assert lib.plqcd_hip_hop_tuning(0, 1, 1, 256, 4, 1, -1, 2) != 0
+    assert lib.plqcd_hip_hop_tuning(0, 1, 1, 256, 4, 3, 0, 0) != 0
+    assert lib.plqcd_hip_hop_tuning_reset(-1) == 0

@@ -80,7 +80,6 @@ def test_cg_batch_equals_alone_bitwise(hp):
 def test_cg_rhs_blocked_equals_alone_bitwise(hp):
     """A 4-RHS solve whose hopping kernels run 4 RHS per thread (norm and
     axpy epilogues included) follows each RHS's lone iterates bitwise."""
-    import ctypes as C
     from plqcd_amd import capi
     from plqcd_amd import solver as sv
     lib = capi.lib()
@@ -88,13 +87,8 @@ def test_cg_rhs_blocked_equals_alone_bitwise(hp):
     kaps, tols = [0.12, 0.124, 0.1245, 0.11], [1e-8, 1e-10, 1e-12, 1e-9]
     lat, g, b, u, bs = setup(hp, L, nrhs=4)
     gs = hp.Gauge(lat, 2, 12).upload(u)
-    saved = []
-    for prec in range(3):
-        t = (C.c_int * 5)()
-        lib.plqcd_hip_hop_get_tuning(prec, t)
-        saved.append(tuple(t))
     try:
-        assert lib.plqcd_hip_hop_tuning(-1, 2, 8, 128, 0, 4) == 0
+        assert lib.plqcd_hip_hop_tuning(-1, 2, 8, 128, 0, 4, 0, 0) == 0
         for sloppy in (None, gs):
             x = hp.Field(lat, nrhs=4)
             st = sv.cg_solve(g, b, x, kaps, tols, sloppy=sloppy, delta=0.1 if sloppy else 0.0)
@@ -107,8 +101,36 @@ def test_cg_rhs_blocked_equals_alone_bitwise(hp):
                 assert np.array_equal(x1.download(), x.download(r))
                 assert st[r]["converged"]
     finally:
-        for prec in range(3):
-            lib.plqcd_hip_hop_tuning(prec, *saved[prec])
+        assert lib.plqcd_hip_hop_tuning_reset(-1) == 0
+
+
+def test_cg_tiled_kernel_equals_default_bitwise(hp):
+    """A one-RHS solve whose hopping kernels run the LDS-tiled t-march (with
+    the M, norm and CG-update epilogues) follows the gather kernel's iterates
+    bitwise, in FP64 and mixed precision."""
+    from plqcd_amd import capi
+    from plqcd_amd import solver as sv
+    lib = capi.lib()
+    L = (8, 8, 8, 8)
+    lat, g, b, u, bs = setup(hp, L, nrhs=1)
+    gs = hp.Gauge(lat, 1, 12).upload(u)
+    results = []
+    try:
+        for tuning in ((1, 1, 256, 0, 1, 0, 0), (1, 1, 256, 4, 1, 4, 4), (0, 1, 256, 8, 1, 8, 2)):
+            assert lib.plqcd_hip_hop_tuning(-1, *tuning) == 0
+            row = []
+            for sloppy in (None, gs):
+                x = hp.Field(lat)
+                st = sv.cg_solve(g, b, x, KAPPA, 1e-10, sloppy=sloppy)[0]
+                assert st["converged"]
+                row.append((st["iterations"], st["true_residual"], x.download()))
+                x.close()
+            results.append(row)
+    finally:
+        assert lib.plqcd_hip_hop_tuning_reset(-1) == 0
+    for row in results[1:]:
+        for (i0, r0, x0), (i1, r1, x1) in zip(results[0], row):
+            assert i0 == i1 and r0 == r1 and np.array_equal(x0, x1)
 
 
 def test_cg_graph_replay_equals_eager(hp):
