@@ -12,6 +12,10 @@ boundaries, Eq. 2.1 (wilson mode).  Prints one JSON line.
   cpu_baseline = the reference's CPU hopping matrix (compiled ASSYMBLY eo+oe,
               OpenMP) or, where that binary is absent, the oracle's C/OpenMP
               restatement, timed on this host on a bounded sample
+  formats   = the same H with FP64 12-real links, FP32 18/12-real and FP16
+              12-real storage (200 launches each)
+  solver    = plain FP64 CG on one RHS against the mixed-precision CG on 12
+              RHS (FP16 storage, FP64 reliable updates), true residual 1e-10
 """
 from __future__ import annotations
 
@@ -107,6 +111,24 @@ def cpu_baseline(L, seconds: float):
     t = oracle.bench(q, u, L, n, oracle.MODE_WILSON, cores)
     return {"value": round(n * V * FLOP_PER_SITE / t / 1e9, 3), "unit": "GFLOP/s", "cores": cores, "kind": "port",
             "sample": f"C/OpenMP restatement eo+oe ({cores} threads), 32^3x64 FP64, {n} applications of H in {t:.1f}s"}
+
+
+def formats_run(L, mode, steps=200, warmup=10):
+    """The same application of H in the other storage formats (FP64 with 12-real
+    links, FP32 with 18- and 12-real links, FP16 with 12-real links), default
+    launch shapes; per format: time per launch, GFLOP/s and compulsory GB/s."""
+    V = int(np.prod(L))
+    out = {}
+    for prec, links in ((0, 12), (1, 18), (1, 12), (2, 12)):
+        ms = gpu_run(L, prec, links, steps, warmup, mode)
+        t = ms / 1e3 / steps
+        gbs = BYTES_PER_SITE[(prec, links)] * V / t / 1e9
+        out[f"{PREC_NAMES[prec]}_l{links}"] = {"us_per_launch": round(t * 1e6, 2),
+                                               "gflops": round(FLOP_PER_SITE * V / t / 1e9, 1),
+                                               "compulsory_bytes_per_site": BYTES_PER_SITE[(prec, links)],
+                                               "compulsory_GBps": round(gbs, 1),
+                                               "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 4)}
+    return out
 
 
 def solve_compare(L, nrhs=12, sloppy=(2, 12), kappa=0.124, tol=1e-10):
@@ -355,7 +377,7 @@ def serve_suite(duration: float, rate: float, seed: int = 7):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--prec", choices=["f64", "f32", "f16"], default="f64")
     ap.add_argument("--links", type=int, default=None)
@@ -364,6 +386,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solver", action="store_true")
+    ap.add_argument("--no-formats", action="store_true", help="skip the FP32/FP16 storage-format timings")
     ap.add_argument("--suite", choices=["hop", "solver", "serve", "calib"], default="hop",
                     help="hop: the headline line (default); solver/serve: the extended studies, one JSON line each")
     ap.add_argument("--duration", type=float, default=60.0, help="serve suite: seconds of Poisson arrivals")
@@ -417,6 +440,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(tag)},
         "cpu_baseline": cpu,
+        "formats": None if (args.no_formats or prec != 0) else formats_run(L, mode),
         "solver": None if args.no_solver else solve_compare(L),
     }
     print(json.dumps(rec), flush=True)

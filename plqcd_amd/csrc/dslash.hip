@@ -7,11 +7,15 @@
 
 #include "dslash.h"
 
-// Defaults per storage precision (F64, F32, F16), from sweeps at 32^3 x 64:
-// one RHS, and several RHS per launch (R lanes per site favour the gather
-// kernel for 16-bit storage and XCDs sweeping the lattice together).
+// Defaults per storage precision (F64, F32, F16), from sweeps at 32^3 x 64.
+// One RHS: the LDS-tiled t-march for F64 (tiles of 32 x 4 x 1 sites, 32
+// t-slices: 434 us against 514 us for the untiled t-march) and F32 (32 x 4 x 4,
+// 8 t-slices: 174 against 217 us), the gather kernel for F16 (129 us, the tile
+// 136 us).  Shapes that do not fit a lattice fall back to the untiled t-march
+// or the gather kernel.  Several RHS per launch: R lanes per site favour the
+// gather kernel for 16-bit storage and XCDs sweeping the lattice together.
 namespace {
-const HopTuning kOneRhs[3] = {{1, 1, 256, 16, 1}, {1, 1, 256, 16, 1}, {1, 1, 64, 4, 1}};
+const HopTuning kOneRhs[3] = {{1, 1, 256, 32, 1, 4, 1}, {2, 8, 256, 8, 1, 4, 4}, {1, 1, 128, 0, 1, 0, 0}};
 const HopTuning kMultiRhs[3] = {{2, 8, 256, 16, 1}, {2, 8, 256, 4, 1}, {2, 8, 256, 0, 1}};
 bool g_tuning_set[3] = {false, false, false};  // set by plqcd_hip_hop_tuning
 }  // namespace
