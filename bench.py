@@ -33,6 +33,10 @@ import numpy as np
 
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
+# The serving suite runs one HIP stream per solver engine and per transfer
+# thread; with HIP's default of 4 hardware queues per process unrelated
+# streams share queues and wait for each other.  Read at HIP initialisation.
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
 
 METRIC = "hopping GFLOP/s & %HBM roofline at 32^3x64; batched mixed vs FP64 solves/sec @1e-10"
 HBM_PEAK_GBS = 8000.0
@@ -350,8 +354,10 @@ def serve_suite(duration: float, rate: float, seed: int = 7):
     lat_s = np.array([d["end_time"] - d["submit_time"] for d in done.values()])
     nrhs = sum(submitted[i]["nrhs"] for i in done)
     ok = all(d["status"] == 0 and d["converged"] for d in done.values())
-    # one-at-a-time plain FP64 baseline: seconds per single-RHS solve, per volume
-    per_rhs = {}
+    # one-at-a-time plain FP64 baseline, per volume: seconds per single-RHS
+    # solve with the fields resident, and with the host transfers of its
+    # source and solution (what the server's requests include)
+    per_rhs, per_rhs_io = {}, {}
     for L in vols:
         V = int(np.prod(L))
         la = hp.Lattice(L)
@@ -362,16 +368,29 @@ def serve_suite(duration: float, rate: float, seed: int = 7):
         t1 = time.perf_counter()
         sv.cg_solve(g, b, x, 0.1225, 1e-10)
         per_rhs[L] = time.perf_counter() - t1
+        xh = np.empty_like(pool[L][0])
+        t1 = time.perf_counter()
+        b.upload(pool[L][1])
+        sv.cg_solve(g, b, x, 0.1225, 1e-10)
+        xh[...] = x.download()
+        per_rhs_io[L] = time.perf_counter() - t1
+        del xh
         x.close(); b.close(); g.close(); la.close()
     base_seconds = sum(per_rhs[submitted[i]["L"]] * submitted[i]["nrhs"] for i in done)
+    base_io_seconds = sum(per_rhs_io[submitted[i]["L"]] * submitted[i]["nrhs"] for i in done)
     emit({"suite": "serve", "case": "poisson_mixed_volumes", "duration_s": duration, "rate_per_s": rate,
           "requests": len(done), "rhs_solved": nrhs, "all_converged": ok, "wall_s": round(wall, 2),
           "p50_latency_s": round(float(np.percentile(lat_s, 50)), 4),
           "p99_latency_s": round(float(np.percentile(lat_s, 99)), 4),
           "solves_per_s": round(nrhs / wall, 3),
           "fp64_one_at_a_time_solves_per_s": round(nrhs / base_seconds, 3),
+          "fp64_one_at_a_time_with_host_io_solves_per_s": round(nrhs / base_io_seconds, 3),
           "fp64_seconds_per_rhs": {"x".join(map(str, L)): round(v, 4) for L, v in per_rhs.items()},
-          "hbm_budget_gb": round(info["budget"] / 2**30, 1), "failed": info["failed"]})
+          "fp64_seconds_per_rhs_with_host_io": {"x".join(map(str, L)): round(v, 4) for L, v in per_rhs_io.items()},
+          "hbm_budget_gb": round(info["budget"] / 2**30, 1), "failed": info["failed"],
+          "engines_created": info["engines_created"], "evictions": info["evictions"],
+          "host_transfer_gb": round(info["transfer_bytes"] / 1e9, 1),
+          "host_transfer_thread_seconds": round(info["transfer_seconds"], 1)})
 
 
 def main():

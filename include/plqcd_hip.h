@@ -256,8 +256,10 @@ typedef struct {
    int sloppy_prec;    /* inner CG precision: PLQCD_PREC_F64 / F32 / F16         */
    int sloppy_links;   /* 18 or 12 (F16: 12)                                     */
    int slots;          /* right-hand sides per engine, 1..32                     */
-   size_t hbm_budget;  /* bytes the server may allocate (0: free HBM - 4 GiB)    */
+   size_t hbm_budget;  /* bytes the server may allocate (0: free HBM - 8 GiB)    */
    int use_graph;      /* replay captured CG iterations                          */
+   int transfer_threads; /* concurrent host transfers of sources and solutions
+                            (0: 8)                                               */
 } plqcd_serve_config;
 
 typedef struct {
@@ -289,8 +291,13 @@ int plqcd_hip_server_submit(plqcd_hip_server *s, const plqcd_serve_request *req,
 int plqcd_hip_server_poll(plqcd_hip_server *s, plqcd_serve_result *out, int max, int *n);
 /* Waits until at least `count` results are ready to poll. */
 int plqcd_hip_server_wait(plqcd_hip_server *s, int count, double timeout_s);
-/* out = {budget bytes, bytes in use, cached (lattice, gauge) entries, internal failures} */
-int plqcd_hip_server_info(plqcd_hip_server *s, double out[4]);
+/* out = {budget bytes, bytes in use, cached (lattice, gauge) entries, internal
+ * failures, engines created, cache entries evicted, host bytes transferred
+ * (sources and solutions), seconds the transfer threads spent transferring}.
+ * Serving runs one HIP stream per engine and per transfer thread: give the
+ * process enough hardware queues (GPU_MAX_HW_QUEUES, e.g. 16) or unrelated
+ * streams share queues and wait for each other. */
+int plqcd_hip_server_info(plqcd_hip_server *s, double out[8]);
 /* Finishes the queued requests, then frees everything. */
 void plqcd_hip_server_destroy(plqcd_hip_server *s);
 

@@ -34,7 +34,7 @@ class CgStats(C.Structure):
 
 class ServeConfig(C.Structure):
     _fields_ = [("device", C.c_int), ("sloppy_prec", C.c_int), ("sloppy_links", C.c_int), ("slots", C.c_int),
-                ("hbm_budget", C.c_size_t), ("use_graph", C.c_int)]
+                ("hbm_budget", C.c_size_t), ("use_graph", C.c_int), ("transfer_threads", C.c_int)]
 
 
 class ServeRequest(C.Structure):

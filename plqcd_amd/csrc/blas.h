@@ -8,23 +8,30 @@
 
 namespace plqcd {
 
+// A window of right-hand sides: RHS r0 .. r0+nr-1 of the fields (nr = 0: all).
+struct Window {
+   int r0 = 0, nr = 0;
+};
+
 size_t partial_doubles(const plqcd_hip_field *f);
 int blas_norm_finish(const plqcd_hip_field *f, double *partial, double *out);  // overwrites partial
 
 // p = r + beta p
-int blas_xpay(const plqcd_hip_field *r, const double *beta, plqcd_hip_field *p);
+int blas_xpay(const plqcd_hip_field *r, const double *beta, plqcd_hip_field *p, const Window &w = Window{});
 // The masked kernels below touch only the RHS r with mask[r] != 0 (all when
-// mask is null); the chunk sums of untouched RHS are written as 0.
+// mask is null); the chunk sums of untouched RHS are written as 0.  With a
+// window, RHS outside it are not touched at all (their chunk sums neither).
 // r = b - t ; partial <- |r|^2   (F64)
 int blas_residual(const plqcd_hip_field *b, const plqcd_hip_field *t, plqcd_hip_field *r, double *partial,
-                  const int *mask);
+                  const int *mask, const Window &w = Window{});
 // y += x ; x = 0 if zero_x   (any precisions)
-int blas_accum(plqcd_hip_field *y, plqcd_hip_field *x, int zero_x, const int *mask);
+int blas_accum(plqcd_hip_field *y, plqcd_hip_field *x, int zero_x, const int *mask, const Window &w = Window{});
 // y = x ; partial <- |y|^2   (any precisions)
-int blas_copy(plqcd_hip_field *y, const plqcd_hip_field *x, double *partial, const int *mask);
+int blas_copy(plqcd_hip_field *y, const plqcd_hip_field *x, double *partial, const int *mask,
+              const Window &w = Window{});
 // y = x - c[r] y on the sites of `parity` only
 int blas_xmcy_parity(const plqcd_hip_field *x, plqcd_hip_field *y, const double *c, int parity);
 // y = 0 for RHS r with mask[r] != 0 (every RHS when mask is null)
-int blas_zero(plqcd_hip_field *y, const int *mask);
+int blas_zero(plqcd_hip_field *y, const int *mask, const Window &w = Window{});
 
 }  // namespace plqcd

@@ -10,18 +10,20 @@ namespace {
 
 constexpr int kBlock = 256;
 
-// Lanes (js, r): tid = js * R + r; a block covers spb sites of one parity.
+// Lanes (js, lane): tid = js * R + lane; a block covers spb sites of one
+// parity, and lane k works on the field's RHS r = r0 + k.
 struct BlasArgs {
    int Vh, R, spb, nbp;  // nbp: blocks per parity
    int nchunk;
-   double *partial;      // [R][2*nchunk] or null
+   int r0;               // first RHS of the launch's window
+   double *partial;      // [nrhs][2*nchunk] or null
 };
 
 __device__ __forceinline__ bool blas_site(const BlasArgs &a, int &p, int &j, int &js, int &r)
 {
    const int blk = blockIdx.x;
    p = blk / a.nbp;
-   r = threadIdx.x % a.R;
+   r = a.r0 + threadIdx.x % a.R;
    js = threadIdx.x / a.R;
    j = (blk - p * a.nbp) * a.spb + js;
    return j < a.Vh;
@@ -29,7 +31,8 @@ __device__ __forceinline__ bool blas_site(const BlasArgs &a, int &p, int &j, int
 
 __device__ __forceinline__ void blas_reduce(const BlasArgs &a, double v, int p, int j, int js, int r)
 {
-   chunk_reduce<kBlock>(v, js, r, a.R, p * a.nchunk + j / REDUCE_CHUNK, j < a.Vh, a.partial + (size_t)r * 2 * a.nchunk);
+   chunk_reduce<kBlock>(v, js, r - a.r0, a.R, p * a.nchunk + j / REDUCE_CHUNK, j < a.Vh,
+                        a.partial + (size_t)r * 2 * a.nchunk);
 }
 
 // p = r + beta p
@@ -178,11 +181,12 @@ __global__ __launch_bounds__(256) void k_reduce_fin(const double *partial, size_
    if (t == 0) out[r] = tot;
 }
 
-BlasArgs make_args(const plqcd_hip_field *f, double *partial)
+BlasArgs make_args(const plqcd_hip_field *f, double *partial, const plqcd::Window &w)
 {
    BlasArgs a;
    a.Vh = f->lat->Vh;
-   a.R = f->nrhs;
+   a.R = w.nr > 0 ? w.nr : f->nrhs;
+   a.r0 = w.nr > 0 ? w.r0 : 0;
    a.spb = REDUCE_CHUNK * (kBlock / (REDUCE_CHUNK * a.R));
    a.nbp = (a.Vh + a.spb - 1) / a.spb;
    a.nchunk = plqcd::reduce_chunks(a.Vh);
@@ -190,11 +194,13 @@ BlasArgs make_args(const plqcd_hip_field *f, double *partial)
    return a;
 }
 
-int check_pair(const plqcd_hip_field *a, const plqcd_hip_field *b)
+int check_pair(const plqcd_hip_field *a, const plqcd_hip_field *b, const plqcd::Window &w)
 {
    if (!a || !b) return plqcd::set_error(1, "blas: null field");
    if (a->lat != b->lat || a->nrhs != b->nrhs) return plqcd::set_error(1, "blas: fields of different shapes");
    if (a->nrhs * REDUCE_CHUNK > kBlock) return plqcd::set_error(2, "blas: at most %d right-hand sides", kBlock / REDUCE_CHUNK);
+   if (w.nr < 0 || w.r0 < 0 || (w.nr > 0 && w.r0 + w.nr > a->nrhs))
+      return plqcd::set_error(1, "blas: RHS window %d + %d outside the fields' %d", w.r0, w.nr, a->nrhs);
    return 0;
 }
 
@@ -223,10 +229,12 @@ dim3 block_of(const BlasArgs &a) { return dim3(a.R * a.spb); }
 
 namespace plqcd {
 
-int reduce_partials(double *partial, size_t stride, int first, int n, int R, double *out, hipStream_t st)
+int reduce_partials(double *partial, size_t stride, int first, int n, int R, double *out, hipStream_t st, int r0)
 {
-   if (n < 1 || R < 1) return set_error(1, "reduce_partials: empty reduction");
+   if (n < 1 || R < 1 || r0 < 0) return set_error(1, "reduce_partials: empty reduction");
    const int nseg = (n + kSeg - 1) / kSeg;
+   partial += (size_t)r0 * stride;
+   out += r0;
    hipLaunchKernelGGL(k_reduce_seg, dim3(nseg, R), dim3(256), 0, st, partial, stride, first, n);
    hipLaunchKernelGGL(k_reduce_fin, dim3(R), dim3(256), 0, st, partial, stride, first, nseg, out);
    PLQCD_HIP_TRY(hipGetLastError());
@@ -241,11 +249,11 @@ int blas_norm_finish(const plqcd_hip_field *f, double *partial, double *out)
    return reduce_partials(partial, (size_t)2 * nc, 0, 2 * nc, f->nrhs, out, f->lat->stream);
 }
 
-int blas_xpay(const plqcd_hip_field *r, const double *beta, plqcd_hip_field *p)
+int blas_xpay(const plqcd_hip_field *r, const double *beta, plqcd_hip_field *p, const Window &w)
 {
-   if (int rc = check_pair(r, p)) return rc;
+   if (int rc = check_pair(r, p, w)) return rc;
    if (r->prec != p->prec) return set_error(1, "blas_xpay: precision mismatch");
-   const BlasArgs a = make_args(r, nullptr);
+   const BlasArgs a = make_args(r, nullptr, w);
    hipStream_t st = r->lat->stream;
    return dispatch1(r->prec, [&](auto P) {
       constexpr int PP = decltype(P)::value;
@@ -254,12 +262,12 @@ int blas_xpay(const plqcd_hip_field *r, const double *beta, plqcd_hip_field *p)
 }
 
 int blas_residual(const plqcd_hip_field *b, const plqcd_hip_field *t, plqcd_hip_field *r, double *partial,
-                  const int *mask)
+                  const int *mask, const Window &w)
 {
-   if (int rc = check_pair(b, t) | check_pair(b, r)) return rc;
+   if (int rc = check_pair(b, t, w) | check_pair(b, r, w)) return rc;
    if (b->prec != PLQCD_PREC_F64 || t->prec != PLQCD_PREC_F64 || r->prec != PLQCD_PREC_F64)
       return set_error(1, "blas_residual: F64 fields expected");
-   const BlasArgs a = make_args(b, partial);
+   const BlasArgs a = make_args(b, partial, w);
    hipLaunchKernelGGL((k_residual<PLQCD_PREC_F64, PLQCD_PREC_F64, PLQCD_PREC_F64>), grid_of(a), block_of(a), 0,
                       b->lat->stream, spinor_io<PLQCD_PREC_F64>(b), spinor_io<PLQCD_PREC_F64>(t),
                       spinor_io<PLQCD_PREC_F64>(r), mask, a);
@@ -267,10 +275,10 @@ int blas_residual(const plqcd_hip_field *b, const plqcd_hip_field *t, plqcd_hip_
    return 0;
 }
 
-int blas_accum(plqcd_hip_field *y, plqcd_hip_field *x, int zero_x, const int *mask)
+int blas_accum(plqcd_hip_field *y, plqcd_hip_field *x, int zero_x, const int *mask, const Window &w)
 {
-   if (int rc = check_pair(y, x)) return rc;
-   const BlasArgs a = make_args(y, nullptr);
+   if (int rc = check_pair(y, x, w)) return rc;
+   const BlasArgs a = make_args(y, nullptr, w);
    hipStream_t st = y->lat->stream;
    return dispatch2(y->prec, x->prec, [&](auto PY, auto PX) {
       constexpr int A = decltype(PY)::value, B = decltype(PX)::value;
@@ -278,10 +286,10 @@ int blas_accum(plqcd_hip_field *y, plqcd_hip_field *x, int zero_x, const int *ma
    });
 }
 
-int blas_copy(plqcd_hip_field *y, const plqcd_hip_field *x, double *partial, const int *mask)
+int blas_copy(plqcd_hip_field *y, const plqcd_hip_field *x, double *partial, const int *mask, const Window &w)
 {
-   if (int rc = check_pair(y, x)) return rc;
-   const BlasArgs a = make_args(y, partial);
+   if (int rc = check_pair(y, x, w)) return rc;
+   const BlasArgs a = make_args(y, partial, w);
    hipStream_t st = y->lat->stream;
    return dispatch2(y->prec, x->prec, [&](auto PY, auto PX) {
       constexpr int A = decltype(PY)::value, B = decltype(PX)::value;
@@ -291,9 +299,9 @@ int blas_copy(plqcd_hip_field *y, const plqcd_hip_field *x, double *partial, con
 
 int blas_xmcy_parity(const plqcd_hip_field *x, plqcd_hip_field *y, const double *c, int parity)
 {
-   if (int rc = check_pair(x, y)) return rc;
+   if (int rc = check_pair(x, y, Window{})) return rc;
    if (x->prec != y->prec) return set_error(1, "blas_xmcy_parity: precision mismatch");
-   const BlasArgs a = make_args(y, nullptr);
+   const BlasArgs a = make_args(y, nullptr, Window{});
    hipStream_t st = y->lat->stream;
    return dispatch1(y->prec, [&](auto P) {
       constexpr int PP = decltype(P)::value;
@@ -302,10 +310,10 @@ int blas_xmcy_parity(const plqcd_hip_field *x, plqcd_hip_field *y, const double 
    });
 }
 
-int blas_zero(plqcd_hip_field *y, const int *mask)
+int blas_zero(plqcd_hip_field *y, const int *mask, const Window &w)
 {
-   if (int rc = check_pair(y, y)) return rc;
-   const BlasArgs a = make_args(y, nullptr);
+   if (int rc = check_pair(y, y, w)) return rc;
+   const BlasArgs a = make_args(y, nullptr, w);
    hipStream_t st = y->lat->stream;
    return dispatch1(y->prec, [&](auto P) {
       constexpr int PP = decltype(P)::value;

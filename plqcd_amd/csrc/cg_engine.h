@@ -12,7 +12,7 @@
 
 #include <vector>
 
-#include "device_common.h"
+#include "blas.h"
 
 namespace plqcd {
 
@@ -47,13 +47,32 @@ class CgEngine {
    // kappa/tol/max_iter per listed slot.  Asynchronous except for one |b|^2
    // readback.
    int admit(const std::vector<int> &which, const double *kappa, const double *tol, const int *max_iter);
-   // Uploads a host source into slot s and admits it.
-   int admit_host(int s, const spinor *b_host, double kappa, double tol, int max_iter);
-   // Enqueues one iteration of all active slots (asynchronous).
+   // Uploads host sources into the listed slots and admits them together.
+   int admit_host(const std::vector<int> &which, const std::vector<const spinor *> &b_host, double kappa, double tol,
+                  int max_iter);
+   // Enqueues one iteration of all active slots (asynchronous).  The kernels
+   // cover the slots up to the highest active one; each such width has its
+   // own captured graph.
    int launch();
    // Waits for the iteration, runs the reliable updates it calls for and
    // appends the slots that finished to `finished`.
    int finish(std::vector<int> &finished);
+   // The same without blocking, for a loop that drives several engines:
+   //   launch(); ... ready() ... begin_finish(updating) (after the iteration
+   //   completed; enqueues the reliable updates it calls for and sets
+   //   `updating`); ... ready() ... end_finish(finished) (after those updates,
+   //   or at once when there were none).  Slots are admitted only between
+   //   end_finish and the next launch.
+   bool ready() const;  // the engine's last enqueued work has completed
+   // Captures the iteration graphs of every window width now (with
+   // use_graph), so that no stream capture happens later: another thread may
+   // then wait on events of the engine's stream at any time.
+   int capture_all();
+   // Stops every active slot unconverged (after a failed launch or update)
+   // and lists it in `finished`.
+   void abort_active(std::vector<int> &finished);
+   int begin_finish(bool &updating);
+   int end_finish(std::vector<int> &finished);
    // Downloads x of a finished slot into host memory (may be null), returns
    // its stats and frees the slot.
    int retire(int s, spinor *x_host, plqcd_cg_stats *st);
@@ -61,9 +80,9 @@ class CgEngine {
   private:
    CgEngine() = default;
    int alloc(int sloppy);
-   int iteration();
+   int iteration(int nr);
    int reliable_update();
-   int capture();
+   int capture(int nr);
 
    plqcd_hip_lattice *lat_ = nullptr;
    const plqcd_hip_gauge *g_ = nullptr, *gs_ = nullptr;
@@ -78,8 +97,13 @@ class CgEngine {
    double *scal_ = nullptr;  // kappa, rr, rr_new, pap, alpha, beta, tmp (7 x S)
    int *imask_ = nullptr;    // active, update (2 x S)
    double *host_ = nullptr;  // pinned: rr_new, tmp (2 x S)
-   hipGraph_t graph_def_ = nullptr;
-   hipGraphExec_t graph_ = nullptr;
+   int *hmask_ = nullptr;    // pinned: active, update (2 x S), the sources of their uploads
+   hipEvent_t ev_ = nullptr; // recorded after each enqueued iteration / reliable update
+   struct Graph {
+      hipGraph_t def = nullptr;
+      hipGraphExec_t exec = nullptr;
+   };
+   std::vector<Graph> graphs_;  // by window width 1..S
    std::vector<CgSlot> slot_;
    std::vector<int> active_, update_;
 

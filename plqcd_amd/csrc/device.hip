@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cstring>
 #include <new>
+#include <type_traits>
 
 #include "fields.h"
 
@@ -337,6 +338,183 @@ int plqcd::field_download_range(const plqcd_hip_field *f, int rhs, spinor *host,
           hipMemcpyAsync(host + i0, lat->stage, (size_t)n * sizeof(spinor), hipMemcpyDeviceToHost, lat->stream));
    }
    PLQCD_HIP_TRY(hipStreamSynchronize(lat->stream));
+   return 0;
+}
+
+namespace {
+constexpr size_t kStagerChunk = 32u << 20;  // bytes per pinned / device buffer
+
+template <class F>
+void dispatch_prec(int prec, F f)
+{
+   switch (prec) {
+      case PLQCD_PREC_F64: f(std::integral_constant<int, PLQCD_PREC_F64>{}); break;
+      case PLQCD_PREC_F32: f(std::integral_constant<int, PLQCD_PREC_F32>{}); break;
+      default: f(std::integral_constant<int, PLQCD_PREC_F16>{});
+   }
+}
+}  // namespace
+
+int plqcd::Stager::create(int device, Stager **out)
+{
+   *out = nullptr;
+   Stager *g = new (std::nothrow) Stager();
+   if (!g) return set_error(4, "out of host memory");
+   hipError_t e = hipSetDevice(device);
+   if (e == hipSuccess) e = hipStreamCreateWithFlags(&g->st_, hipStreamNonBlocking);
+   for (int b = 0; b < 2 && e == hipSuccess; b++) {
+      e = hipHostMalloc(&g->pin_[b], kStagerChunk, hipHostMallocDefault);
+      if (e == hipSuccess) e = hipMalloc(&g->dev_[b], kStagerChunk);
+      if (e == hipSuccess) e = hipEventCreateWithFlags(&g->ev_[b], hipEventDisableTiming);
+   }
+   if (e != hipSuccess) {
+      delete g;
+      return set_error(10, "stager: %s", hipGetErrorString(e));
+   }
+   *out = g;
+   return 0;
+}
+
+plqcd::Stager::~Stager()
+{
+   if (st_) (void)hipStreamSynchronize(st_);
+   for (int b = 0; b < 2; b++) {
+      if (ev_[b]) (void)hipEventDestroy(ev_[b]);
+      if (dev_[b]) (void)hipFree(dev_[b]);
+      if (pin_[b]) (void)hipHostFree(pin_[b]);
+   }
+   if (st_) (void)hipStreamDestroy(st_);
+}
+
+int plqcd::Stager::upload(plqcd_hip_field *f, int rhs, const spinor *host)
+{
+   if (int rc = check_rhs(f, rhs)) return rc;
+   if (!host) return set_error(1, "stager: null host pointer");
+   const int V = f->lat->V, Vh = f->lat->Vh;
+   const int per = (int)(kStagerChunk / sizeof(spinor));
+   bool used[2] = {false, false};
+   for (int i0 = 0, c = 0; i0 < V; i0 += per, c++) {
+      const int b = c & 1, n = std::min(per, V - i0);
+      if (used[b]) PLQCD_HIP_TRY(hipEventSynchronize(ev_[b]));  // buffer b's previous chunk is packed
+      std::memcpy(pin_[b], host + i0, (size_t)n * sizeof(spinor));
+      PLQCD_HIP_TRY(hipMemcpyAsync(dev_[b], pin_[b], (size_t)n * sizeof(spinor), hipMemcpyHostToDevice, st_));
+      const d2 *src = static_cast<const d2 *>(dev_[b]);
+      const dim3 grid((n + kPackBlock - 1) / kPackBlock), block(kPackBlock);
+      dispatch_prec(f->prec, [&](auto P) {
+         constexpr int PP = decltype(P)::value;
+         hipLaunchKernelGGL(pack_spinor<PP>, grid, block, 0, st_, src, spinor_io<PP>(f), Vh, rhs, i0, n);
+      });
+      PLQCD_HIP_TRY(hipGetLastError());
+      PLQCD_HIP_TRY(hipEventRecord(ev_[b], st_));
+      used[b] = true;
+   }
+   PLQCD_HIP_TRY(hipStreamSynchronize(st_));
+   return 0;
+}
+
+int plqcd::Stager::download(const plqcd_hip_field *f, int rhs, spinor *host)
+{
+   if (int rc = check_rhs(f, rhs)) return rc;
+   if (!host) return set_error(1, "stager: null host pointer");
+   const int V = f->lat->V, Vh = f->lat->Vh;
+   const int per = (int)(kStagerChunk / sizeof(spinor));
+   int pend_i0 = -1, pend_n = 0, pend_b = 0;  // chunk whose D2H is in flight
+   for (int i0 = 0, c = 0; i0 < V; i0 += per, c++) {
+      const int b = c & 1, n = std::min(per, V - i0);
+      d2 *dst = static_cast<d2 *>(dev_[b]);
+      const dim3 grid((n + kPackBlock - 1) / kPackBlock), block(kPackBlock);
+      dispatch_prec(f->prec, [&](auto P) {
+         constexpr int PP = decltype(P)::value;
+         hipLaunchKernelGGL(unpack_spinor<PP>, grid, block, 0, st_, dst, spinor_io<PP>(f), Vh, rhs, i0, n);
+      });
+      PLQCD_HIP_TRY(hipGetLastError());
+      PLQCD_HIP_TRY(hipMemcpyAsync(pin_[b], dev_[b], (size_t)n * sizeof(spinor), hipMemcpyDeviceToHost, st_));
+      PLQCD_HIP_TRY(hipEventRecord(ev_[b], st_));
+      if (pend_i0 >= 0) {  // copy out the previous chunk while this one is in flight
+         PLQCD_HIP_TRY(hipEventSynchronize(ev_[pend_b]));
+         std::memcpy(host + pend_i0, pin_[pend_b], (size_t)pend_n * sizeof(spinor));
+      }
+      pend_i0 = i0;
+      pend_n = n;
+      pend_b = b;
+   }
+   if (pend_i0 >= 0) {
+      PLQCD_HIP_TRY(hipEventSynchronize(ev_[pend_b]));
+      std::memcpy(host + pend_i0, pin_[pend_b], (size_t)pend_n * sizeof(spinor));
+   }
+   return 0;
+}
+
+int plqcd::Stager::upload_raw(void *dev, const void *host, size_t bytes)
+{
+   if (!dev || !host) return set_error(1, "stager: null pointer");
+   bool used[2] = {false, false};
+   for (size_t o = 0, c = 0; o < bytes; o += kStagerChunk, c++) {
+      const int b = (int)(c & 1);
+      const size_t n = std::min(kStagerChunk, bytes - o);
+      if (used[b]) PLQCD_HIP_TRY(hipEventSynchronize(ev_[b]));  // buffer b's previous chunk has left
+      std::memcpy(pin_[b], static_cast<const char *>(host) + o, n);
+      PLQCD_HIP_TRY(hipMemcpyAsync(static_cast<char *>(dev) + o, pin_[b], n, hipMemcpyHostToDevice, st_));
+      PLQCD_HIP_TRY(hipEventRecord(ev_[b], st_));
+      used[b] = true;
+   }
+   PLQCD_HIP_TRY(hipStreamSynchronize(st_));
+   return 0;
+}
+
+int plqcd::Stager::download_raw(void *host, const void *dev, size_t bytes, hipEvent_t after)
+{
+   if (!dev || !host) return set_error(1, "stager: null pointer");
+   // a host-side wait: a stream wait on `after` would tie this stream to the
+   // other stream, which fails while that stream is being captured
+   if (after) PLQCD_HIP_TRY(hipEventSynchronize(after));
+   size_t pend_o = 0, pend_n = 0;
+   int pend_b = -1;
+   for (size_t o = 0, c = 0; o < bytes; o += kStagerChunk, c++) {
+      const int b = (int)(c & 1);
+      const size_t n = std::min(kStagerChunk, bytes - o);
+      PLQCD_HIP_TRY(hipMemcpyAsync(pin_[b], static_cast<const char *>(dev) + o, n, hipMemcpyDeviceToHost, st_));
+      PLQCD_HIP_TRY(hipEventRecord(ev_[b], st_));
+      if (pend_b >= 0) {  // copy out the previous chunk while this one is in flight
+         PLQCD_HIP_TRY(hipEventSynchronize(ev_[pend_b]));
+         std::memcpy(static_cast<char *>(host) + pend_o, pin_[pend_b], pend_n);
+      }
+      pend_o = o;
+      pend_n = n;
+      pend_b = b;
+   }
+   if (pend_b >= 0) {
+      PLQCD_HIP_TRY(hipEventSynchronize(ev_[pend_b]));
+      std::memcpy(static_cast<char *>(host) + pend_o, pin_[pend_b], pend_n);
+   }
+   return 0;
+}
+
+int plqcd::pack_rhs(plqcd_hip_field *f, int rhs, const void *dev_aos, hipStream_t st)
+{
+   if (int rc = check_rhs(f, rhs)) return rc;
+   const int V = f->lat->V, Vh = f->lat->Vh;
+   const d2 *src = static_cast<const d2 *>(dev_aos);
+   const dim3 grid((V + kPackBlock - 1) / kPackBlock), block(kPackBlock);
+   dispatch_prec(f->prec, [&](auto P) {
+      constexpr int PP = decltype(P)::value;
+      hipLaunchKernelGGL(pack_spinor<PP>, grid, block, 0, st, src, spinor_io<PP>(f), Vh, rhs, 0, V);
+   });
+   PLQCD_HIP_TRY(hipGetLastError());
+   return 0;
+}
+
+int plqcd::unpack_rhs(const plqcd_hip_field *f, int rhs, void *dev_aos, hipStream_t st)
+{
+   if (int rc = check_rhs(f, rhs)) return rc;
+   const int V = f->lat->V, Vh = f->lat->Vh;
+   d2 *dst = static_cast<d2 *>(dev_aos);
+   const dim3 grid((V + kPackBlock - 1) / kPackBlock), block(kPackBlock);
+   dispatch_prec(f->prec, [&](auto P) {
+      constexpr int PP = decltype(P)::value;
+      hipLaunchKernelGGL(unpack_spinor<PP>, grid, block, 0, st, dst, spinor_io<PP>(f), Vh, rhs, 0, V);
+   });
+   PLQCD_HIP_TRY(hipGetLastError());
    return 0;
 }
 

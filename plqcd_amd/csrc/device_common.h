@@ -182,3 +182,38 @@ __device__ __forceinline__ uint32_t f2_to_h2(float a, float b)
    __half2 h = __floats2half2_rn(a, b);
    return *reinterpret_cast<uint32_t *>(&h);
 }
+
+namespace plqcd {
+
+// Host <-> device transfers of one RHS of a field on a stream of their own,
+// through two pinned host buffers and two device buffers: the CPU copy of one
+// chunk overlaps the DMA and (un)packing of the previous one, and the field's
+// lattice stream keeps running.  The caller keeps kernels on other streams
+// away from the RHS being transferred.  Each call returns when its transfer
+// is complete; one Stager serves one thread at a time.
+class Stager {
+  public:
+   static int create(int device, Stager **out);
+   ~Stager();
+   int upload(plqcd_hip_field *f, int rhs, const spinor *host);
+   int download(const plqcd_hip_field *f, int rhs, spinor *host);
+   // Plain copies of `bytes` between host memory and a device buffer (DMA
+   // only, no kernels); download first waits (on the host) for `after` when
+   // not null.
+   int upload_raw(void *dev, const void *host, size_t bytes);
+   int download_raw(void *host, const void *dev, size_t bytes, hipEvent_t after);
+
+  private:
+   Stager() = default;
+   hipStream_t st_ = nullptr;
+   void *pin_[2] = {nullptr, nullptr};
+   void *dev_[2] = {nullptr, nullptr};
+   hipEvent_t ev_[2] = {nullptr, nullptr};
+};
+
+// One RHS of a field from / to a device buffer of V spinors in the host
+// (reference) layout, asynchronous on `st`.
+int pack_rhs(plqcd_hip_field *f, int rhs, const void *dev_aos, hipStream_t st);
+int unpack_rhs(const plqcd_hip_field *f, int rhs, void *dev_aos, hipStream_t st);
+
+}  // namespace plqcd

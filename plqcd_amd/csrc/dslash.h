@@ -14,7 +14,8 @@ struct HopArgs {
    int group;    // blocks per XCD run for order 2
    int S;        // sites per parity in one t-slice (t-march)
    int T;        // t-slices per thread (t-march)
-   int R;        // right-hand sides: lanes per site
+   int R;        // right-hand sides per launch: lanes per site
+   int r0;       // RHS of the fields that lane 0 of a site handles (the launch covers r0 .. r0+R-1)
    int spb;      // sites per block
    int ty, tz;   // tile extents in y and z (LDS-tiled kernel)
    int nchunk;   // reduction chunks per parity
@@ -32,10 +33,10 @@ struct HopTuning {
 extern HopTuning g_tuning[3];  // per storage precision
 
 namespace plqcd {
-// Launch shape for `R` right-hand sides before the per-shape cache: the
-// precision's shape set by plqcd_hip_hop_tuning, else a built-in default for
-// R = 1 or for R > 1.
-HopTuning default_tuning(int prec, int R);
+// Launch shape for `R` right-hand sides of storage format (prec, links) before
+// the per-shape cache: the precision's shape set by plqcd_hip_hop_tuning, else
+// a built-in default for R = 1 (per format) or for R > 1 (per precision).
+HopTuning default_tuning(int prec, int links, int R);
 }  // namespace plqcd
 
 HopArgs make_hop_args(const plqcd_hip_lattice *lat, int parity);
@@ -49,7 +50,10 @@ namespace plqcd {
 //   alpha   != null: CG update (with dag, kappa and partial): Ap = in - kappa H^+ in
 //                    is not stored; fuse_x += alpha[r] fuse_p and out -= alpha[r] Ap
 //                    (out holds the residual), whose |out|^2 goes to partial
+//   nr      > 0:     only the right-hand sides r0 .. r0+nr-1 of the fields (kappa,
+//                    alpha and partial stay indexed by the field's RHS)
 struct DslashOp {
+   int r0 = 0, nr = 0;
    int dag = 0;
    const double *kappa = nullptr;
    double *partial = nullptr;

@@ -7,25 +7,47 @@
 
 #include "dslash.h"
 
-// Defaults per storage precision (F64, F32, F16), from sweeps at 32^3 x 64.
-// One RHS: the LDS-tiled t-march for F64 (tiles of 32 x 4 x 1 sites, 32
-// t-slices: 434 us against 514 us for the untiled t-march) and F32 (32 x 4 x 4,
-// 8 t-slices: 174 against 217 us), the gather kernel for F16 (129 us, the tile
+// Built-in launch shapes, from sweeps at 32^3 x 64.  One RHS, per storage
+// format: the LDS-tiled t-march for F64 (tiles of 32 x 4 x 1 sites, 32
+// t-slices: 434 us against 514 us for the untiled t-march) and F32 (32 x 4 x 4
+// with 12-real links: 174 against 217 us; 32 x 4 x 2 with 18-real links, whose
+// 512-lane tile spills registers), the gather kernel for F16 (129 us, the tile
 // 136 us).  Shapes that do not fit a lattice fall back to the untiled t-march
-// or the gather kernel.  Several RHS per launch: R lanes per site favour the
-// gather kernel for 16-bit storage and XCDs sweeping the lattice together.
+// or the gather kernel.  Several RHS per launch, per precision: R lanes per site
+// favour the gather kernel for 16-bit storage and XCDs sweeping the lattice
+// together.
 namespace {
-const HopTuning kOneRhs[3] = {{1, 1, 256, 32, 1, 4, 1}, {2, 8, 256, 8, 1, 4, 4}, {1, 1, 128, 0, 1, 0, 0}};
+struct FormatShape {
+   int prec, links;
+   HopTuning t;
+};
+const FormatShape kOneRhs[5] = {{PLQCD_PREC_F64, 18, {1, 1, 256, 32, 1, 4, 1}},
+                                {PLQCD_PREC_F64, 12, {1, 1, 256, 32, 1, 4, 1}},
+                                {PLQCD_PREC_F32, 18, {2, 8, 256, 8, 1, 4, 2}},
+                                {PLQCD_PREC_F32, 12, {2, 8, 256, 8, 1, 4, 4}},
+                                {PLQCD_PREC_F16, 12, {1, 1, 128, 0, 1, 0, 0}}};
 const HopTuning kMultiRhs[3] = {{2, 8, 256, 16, 1}, {2, 8, 256, 4, 1}, {2, 8, 256, 0, 1}};
 bool g_tuning_set[3] = {false, false, false};  // set by plqcd_hip_hop_tuning
+
+HopTuning one_rhs_shape(int prec, int links)
+{
+   for (const FormatShape &f : kOneRhs)
+      if (f.prec == prec && f.links == links) return f.t;
+   for (const FormatShape &f : kOneRhs)
+      if (f.prec == prec) return f.t;
+   return kOneRhs[0].t;
+}
 }  // namespace
 
-HopTuning g_tuning[3] = {kOneRhs[0], kOneRhs[1], kOneRhs[2]};
+// The one-RHS shapes of the 18-real (F16: 12-real) formats, per precision;
+// plqcd_hip_hop_tuning replaces them.
+HopTuning g_tuning[3] = {one_rhs_shape(PLQCD_PREC_F64, 18), one_rhs_shape(PLQCD_PREC_F32, 18),
+                         one_rhs_shape(PLQCD_PREC_F16, 12)};
 
-HopTuning plqcd::default_tuning(int prec, int R)
+HopTuning plqcd::default_tuning(int prec, int links, int R)
 {
-   if (R <= 1 || g_tuning_set[prec]) return g_tuning[prec];
-   return kMultiRhs[prec];
+   if (g_tuning_set[prec]) return g_tuning[prec];
+   return R <= 1 ? one_rhs_shape(prec, links) : kMultiRhs[prec];
 }
 
 namespace {
@@ -108,6 +130,8 @@ int plqcd::dslash(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip
    if (in->nrhs != out->nrhs) return plqcd::set_error(1, "hop: RHS count mismatch");
    if (parity < PLQCD_PARITY_EVEN || parity > PLQCD_PARITY_ALL) return plqcd::set_error(1, "hop: bad parity %d", parity);
    if (mode != PLQCD_HOP_COMPAT && mode != PLQCD_HOP_WILSON) return plqcd::set_error(1, "hop: bad mode %d", mode);
+   if (op.nr < 0 || op.r0 < 0 || (op.nr > 0 && op.r0 + op.nr > in->nrhs))
+      return plqcd::set_error(1, "hop: RHS window %d + %d outside the fields' %d", op.r0, op.nr, in->nrhs);
    if (op.alpha) {
       for (const plqcd_hip_field *f : {op.fuse_p, static_cast<const plqcd_hip_field *>(op.fuse_x)})
          if (!f || f->lat != g->lat || f->prec != g->prec || f->nrhs != in->nrhs || f == in || f == out)
@@ -152,7 +176,7 @@ extern "C" int plqcd_hip_hop_tuning_reset(int prec)
    if (prec < -1 || prec > PLQCD_PREC_F16) return plqcd::set_error(1, "plqcd_hip_hop_tuning_reset: bad precision");
    for (int p = 0; p < 3; p++)
       if (prec < 0 || p == prec) {
-         g_tuning[p] = kOneRhs[p];
+         g_tuning[p] = one_rhs_shape(p, p == PLQCD_PREC_F16 ? 12 : 18);
          g_tuning_set[p] = false;
       }
    return 0;

@@ -242,7 +242,8 @@ struct CgFuse {
 // CG update at the site instead of storing it: x += alpha_r p, out -= alpha_r
 // Ap (out holds the residual), in the working precision of the kernel (float
 // for F32/F16 storage, double for F64).
-// With NORM the site's |out|^2 joins the deterministic chunk reduction.
+// With NORM the site's |out|^2 joins the deterministic chunk reduction.  `in`
+// and `out` are positioned at the RHS `r`, whose kappa and alpha apply.
 template <int EPI, bool NORM, class SIO>
 __device__ __forceinline__ double epilogue(const SIO &in, const SIO &out, const HopArgs &a, int r, int p, int j,
                                            typename SIO::C acc[12], const typename SIO::C *cen, const CgFuse<SIO> &f)
@@ -304,7 +305,7 @@ __device__ __forceinline__ void hop_block(const SIO &in, const SIO &out, const G
    const int j = blk * a.spb + js;
    double nrm = 0.0;
    if (js < a.spb && j < a.Vh) {
-      const SIO si = in.at_rhs(r), so = out.at_rhs(r);
+      const SIO si = in.at_rhs(a.r0 + r), so = out.at_rhs(a.r0 + r);
       const Nbr n = neighbours(a, p, j);
       C acc[12];
 #pragma unroll
@@ -313,10 +314,10 @@ __device__ __forceinline__ void hop_block(const SIO &in, const SIO &out, const G
       hop_dir<1, MODE, DAG>(si, g, p, j, n, acc);
       hop_dir<2, MODE, DAG>(si, g, p, j, n, acc);
       hop_dir<3, MODE, DAG>(si, g, p, j, n, acc);
-      nrm = epilogue<EPI, NORM>(si, so, a, r, p, j, acc, (const C *)nullptr, f);
+      nrm = epilogue<EPI, NORM>(si, so, a, a.r0 + r, p, j, acc, (const C *)nullptr, f);
    }
    if (NORM) chunk_reduce<BLOCK>(nrm, js, r, a.R, p * a.nchunk + (blk * a.spb + js) / REDUCE_CHUNK,
-                                 j - (js & (REDUCE_CHUNK - 1)) < a.Vh, a.partial + (size_t)r * 2 * a.nchunk);
+                                 j - (js & (REDUCE_CHUNK - 1)) < a.Vh, a.partial + (size_t)(a.r0 + r) * 2 * a.nchunk);
 }
 
 template <int MODE, bool DAG, int EPI, bool NORM, int BLOCK, class SIO, class GIO>
@@ -356,7 +357,7 @@ __device__ __forceinline__ void hop_block_rb(const SIO &in, const SIO &out, cons
       for (int k = 0; k < RB; k++)
 #pragma unroll
          for (int c = 0; c < 12; c++) acc[k][c] = C{0, 0};
-      const SIO si = in.at_rhs(gi * RB);
+      const SIO si = in.at_rhs(a.r0 + gi * RB);
 #define PLQCD_RB_DIR(MU)                                                                      \
    {                                                                                          \
       C u[9];                                                                                 \
@@ -364,7 +365,7 @@ __device__ __forceinline__ void hop_block_rb(const SIO &in, const SIO &out, cons
       _Pragma("unroll") for (int k = 0; k < RB; k++)                                          \
       {                                                                                       \
          C f[12], h[6], r[6];                                                                 \
-         si.at_rhs(gi * RB + k).load(1 - p, n.up[MU], f);                                     \
+         si.at_rhs(a.r0 + gi * RB + k).load(1 - p, n.up[MU], f);                              \
          proj<MU, SF>(f, h);                                                                  \
          mul_u(u, h, r);                                                                      \
          recon<MU, SF, MODE != MODE_COMPAT>(acc[k], r);                                       \
@@ -373,7 +374,7 @@ __device__ __forceinline__ void hop_block_rb(const SIO &in, const SIO &out, cons
       _Pragma("unroll") for (int k = 0; k < RB; k++)                                          \
       {                                                                                       \
          C f[12], h[6], r[6];                                                                 \
-         si.at_rhs(gi * RB + k).load(1 - p, n.dn[MU], f);                                     \
+         si.at_rhs(a.r0 + gi * RB + k).load(1 - p, n.dn[MU], f);                              \
          proj<MU, SB>(f, h);                                                                  \
          mul_udag(u, h, r);                                                                   \
          recon<MU, SB, true>(acc[k], r);                                                      \
@@ -386,14 +387,16 @@ __device__ __forceinline__ void hop_block_rb(const SIO &in, const SIO &out, cons
 #undef PLQCD_RB_DIR
 #pragma unroll
       for (int k = 0; k < RB; k++)
-         nrm[k] = epilogue<EPI, NORM>(in.at_rhs(gi * RB + k), out.at_rhs(gi * RB + k), a, gi * RB + k, p, j, acc[k],
+         nrm[k] = epilogue<EPI, NORM>(in.at_rhs(a.r0 + gi * RB + k), out.at_rhs(a.r0 + gi * RB + k), a,
+                                      a.r0 + gi * RB + k, p, j, acc[k],
                                       (const C *)nullptr, f);
    }
    if (NORM) {
 #pragma unroll
       for (int k = 0; k < RB; k++)
          chunk_reduce<BLOCK>(nrm[k], js, gi, G, p * a.nchunk + (blk * a.spb + js) / REDUCE_CHUNK,
-                             j - (js & (REDUCE_CHUNK - 1)) < a.Vh, a.partial + (size_t)(gi * RB + k) * 2 * a.nchunk);
+                             j - (js & (REDUCE_CHUNK - 1)) < a.Vh,
+                             a.partial + (size_t)(a.r0 + gi * RB + k) * 2 * a.nchunk);
    }
 }
 
@@ -415,18 +418,8 @@ struct BatchItem {
    HopArgs a;
    int first;  // first global block of this item
    int nb;     // blocks of this item
+   int march;  // t-march blocks (a.T t-slices per lane) instead of gather blocks
 };
-
-template <int MODE, int BLOCK, class SIO, class GIO>
-__global__ __launch_bounds__(BLOCK) void hop_batch_kernel(const BatchItem<SIO, GIO> *items, int nitems, int total)
-{
-   for (int b = blockIdx.x; b < total; b += gridDim.x) {
-      int it = 0;
-      while (it + 1 < nitems && b >= items[it + 1].first) it++;
-      const BatchItem<SIO, GIO> &I = items[it];
-      hop_block<MODE, false, 0, false, BLOCK>(I.in, I.out, I.g, I.a, b - I.first, CgFuse<SIO>{});
-   }
-}
 
 // t-march variant (wilson mode, full lattice): a thread owns one (x,y,z)
 // column and walks T consecutive t-slices.  Along t each spinor and each U_t
@@ -438,13 +431,13 @@ __global__ __launch_bounds__(BLOCK) void hop_batch_kernel(const BatchItem<SIO, G
 // the index of the site inside its parity advances by S = L0/2*L1*L2 per
 // t-slice.  Terms are accumulated in the same order as in hop_kernel.
 template <bool DAG, int EPI, bool NORM, int BLOCK, class SIO, class GIO>
-__global__ __launch_bounds__(BLOCK) void hop_tmarch_kernel(SIO in, SIO out, GIO g, HopArgs a, CgFuse<SIO> fuse)
+__device__ __forceinline__ void hop_tmarch_block(const SIO &in, const SIO &out, const GIO &g, const HopArgs &a, int lb,
+                                                 const CgFuse<SIO> &fuse)
 {
    typedef typename SIO::C C;
    constexpr int SF = DAG ? 1 : -1;  // forward projector sign
    const int S = a.S;
    const int nbS = (S + a.spb - 1) / a.spb;
-   int lb = logical_block(a);
    const int q = lb & 1;  // parity of the column's first site
    lb >>= 1;
    const int sb = lb % nbS;
@@ -452,8 +445,8 @@ __global__ __launch_bounds__(BLOCK) void hop_tmarch_kernel(SIO in, SIO out, GIO 
    const int r = threadIdx.x % a.R;
    const int js = threadIdx.x / a.R;
    const int c = sb * a.spb + js;
-   const bool valid = c < S;
-   const SIO si = in.at_rhs(r), so = out.at_rhs(r);
+   const bool valid = js < a.spb && c < S;  // a persistent batch block may have spare lanes
+   const SIO si = in.at_rhs(a.r0 + r), so = out.at_rhs(a.r0 + r);
 
    const int t0 = chunk * a.T;
    const int cc = valid ? c : 0;
@@ -518,12 +511,50 @@ __global__ __launch_bounds__(BLOCK) void hop_tmarch_kernel(SIO in, SIO out, GIO 
             proj<3, -SF>(cen, kk);
             mul_udag(u, kk, b);
          }
-         nrm = epilogue<EPI, NORM>(si, so, a, r, p, j, acc, cen, fuse);
+         nrm = epilogue<EPI, NORM>(si, so, a, a.r0 + r, p, j, acc, cen, fuse);
 #pragma unroll
          for (int m = 0; m < 12; m++) cen[m] = f[m];
       }
       if (NORM) chunk_reduce<BLOCK>(nrm, js, r, a.R, p * a.nchunk + (sb * a.spb + js + base) / REDUCE_CHUNK, valid,
-                                    a.partial + (size_t)r * 2 * a.nchunk);
+                                    a.partial + (size_t)(a.r0 + r) * 2 * a.nchunk);
+   }
+}
+
+template <bool DAG, int EPI, bool NORM, int BLOCK, class SIO, class GIO>
+__global__ __launch_bounds__(BLOCK) void hop_tmarch_kernel(SIO in, SIO out, GIO g, HopArgs a, CgFuse<SIO> fuse)
+{
+   hop_tmarch_block<DAG, EPI, NORM, BLOCK>(in, out, g, a, logical_block(a), fuse);
+}
+
+// The persistent launch over a ragged batch (see BatchItem): each item runs
+// t-march or gather blocks.  Blocks go round-robin to the 8 XCDs; as in
+// block order 2, the XCDs sweep the concatenated logical blocks together, XCD
+// k (blockIdx % 8 == k) taking the k-th run of kGroup consecutive blocks of
+// every 8 kGroup.  The grid is a multiple of 8.  KIND 0: every item runs gather blocks, 1:
+// every item t-march blocks, 2: each item its own kind (the kernel then holds
+// both code paths and needs the registers of the larger).
+template <int MODE, int KIND, int BLOCK, class SIO, class GIO>
+__global__ __launch_bounds__(BLOCK) void hop_batch_kernel(const BatchItem<SIO, GIO> *items, int nitems, int total)
+{
+   constexpr int kGroup = 8;
+   const int xcd = blockIdx.x & 7, per = gridDim.x >> 3;
+   const int span = ((total + 8 * kGroup - 1) / (8 * kGroup)) * kGroup;  // blocks per XCD, rounded up
+   for (int i = blockIdx.x >> 3; i < span; i += per) {
+      const int b = ((i / kGroup) * 8 + xcd) * kGroup + i % kGroup;
+      if (b >= total) break;
+      int it = 0;
+      while (it + 1 < nitems && b >= items[it + 1].first) it++;
+      const BatchItem<SIO, GIO> &I = items[it];
+      if constexpr (MODE == PLQCD_HOP_WILSON && KIND == 1) {
+         hop_tmarch_block<false, 0, false, BLOCK>(I.in, I.out, I.g, I.a, b - I.first, CgFuse<SIO>{});
+      } else if constexpr (MODE == PLQCD_HOP_WILSON && KIND == 2) {
+         if (I.march)
+            hop_tmarch_block<false, 0, false, BLOCK>(I.in, I.out, I.g, I.a, b - I.first, CgFuse<SIO>{});
+         else
+            hop_block<MODE, false, 0, false, BLOCK>(I.in, I.out, I.g, I.a, b - I.first, CgFuse<SIO>{});
+      } else {
+         hop_block<MODE, false, 0, false, BLOCK>(I.in, I.out, I.g, I.a, b - I.first, CgFuse<SIO>{});
+      }
    }
 }
 
@@ -574,6 +605,8 @@ template <bool DAG, int EPI, bool NORM, int BLOCK, class SIO, class GIO>
 __global__ __launch_bounds__(BLOCK) void hop_tile_kernel(SIO in, SIO out, GIO g, HopArgs a, CgFuse<SIO> fuse)
 {
    typedef typename SIO::C C;
+   in = in.at_rhs(a.r0);  // the launch's one RHS
+   out = out.at_rhs(a.r0);
    constexpr int SF = DAG ? 1 : -1;  // forward projector sign
    __shared__ C sm[2][12][BLOCK];
    __shared__ C sb[6][BLOCK];  // the lane's backward t term b (read and written by that lane only)
@@ -680,11 +713,13 @@ __global__ __launch_bounds__(BLOCK) void hop_tile_kernel(SIO in, SIO out, GIO g,
          C cen[12];
 #pragma unroll
          for (int m = 0; m < 12; m++) cen[m] = buf[m][tid];
-         nrm = epilogue<EPI, NORM>(in, out, a, 0, p, j, acc, cen, fuse);
+         nrm = epilogue<EPI, NORM>(in, out, a, a.r0, p, j, acc, cen, fuse);
       } else {
-         nrm = epilogue<EPI, NORM>(in, out, a, 0, p, j, acc, (const C *)nullptr, fuse);
+         nrm = epilogue<EPI, NORM>(in, out, a, a.r0, p, j, acc, (const C *)nullptr, fuse);
       }
-      if (NORM) chunk_reduce<BLOCK>(nrm, lc, 0, 1, p * a.nchunk + j / REDUCE_CHUNK, true, a.partial);
+      if (NORM)
+         chunk_reduce<BLOCK>(nrm, lc, 0, 1, p * a.nchunk + j / REDUCE_CHUNK, true,
+                             a.partial + (size_t)a.r0 * 2 * a.nchunk);
    }
 }
 
@@ -730,9 +765,11 @@ int launch_t(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip_fiel
    typedef GaugeIO<PREC, LINKS> G;
    const S si = spinor_io<PREC>(in), so = spinor_io<PREC>(out);
    const G gg = gauge_io<PREC, LINKS>(g);
-   const int R = in->nrhs;
-   HopTuning tu = plqcd::default_tuning(PREC, R);
-   plqcd::lookup_tuning(plqcd::TuneKey{{lat->L[0], lat->L[1], lat->L[2], lat->L[3]}, PREC, LINKS, in->nrhs}, &tu);
+   // lanes: the window of right-hand sides [op.r0, op.r0 + R) of the fields
+   const int R = op.nr > 0 ? op.nr : in->nrhs;
+   a.r0 = op.nr > 0 ? op.r0 : 0;
+   HopTuning tu = plqcd::default_tuning(PREC, LINKS, R);
+   plqcd::lookup_tuning(plqcd::TuneKey{{lat->L[0], lat->L[1], lat->L[2], lat->L[3]}, PREC, LINKS, R}, &tu);
    // RHS per lane (RHS-blocked gather) when it divides R
    const int rb = (tu.rb == 2 || tu.rb == 4) && R % tu.rb == 0 ? tu.rb : 1;
    const int lanes = R / rb;  // lanes per site
@@ -829,10 +866,16 @@ int launch_batch_t(int n, const plqcd_hip_gauge *const *g, const plqcd_hip_field
       I.in = spinor_io<PREC>(in[i]);
       I.out = spinor_io<PREC>(out[i]);
       I.g = gauge_io<PREC, LINKS>(g[i]);
-      I.a = make_hop_args(g[i]->lat, PLQCD_PARITY_ALL);
+      const plqcd_hip_lattice *lat = g[i]->lat;
+      I.a = make_hop_args(lat, PLQCD_PARITY_ALL);
       I.a.R = in[i]->nrhs;
       I.a.spb = REDUCE_CHUNK * (256 / (REDUCE_CHUNK * I.a.R));
-      I.nb = 2 * ((I.a.Vh + I.a.spb - 1) / I.a.spb);
+      // wilson items march along t with the precision's t-slice count when it
+      // divides the time extent; the others use gather blocks
+      const int T = plqcd::default_tuning(PREC, LINKS, I.a.R).tmarch;
+      I.march = mode == PLQCD_HOP_WILSON && T > 0 && lat->L[3] % T == 0;
+      I.a.T = I.march ? T : 1;
+      I.nb = I.march ? 2 * ((I.a.S + I.a.spb - 1) / I.a.spb) * (lat->L[3] / T) : 2 * ((I.a.Vh + I.a.spb - 1) / I.a.spb);
       I.first = total;
       total += I.nb;
    }
@@ -845,15 +888,17 @@ int launch_batch_t(int n, const plqcd_hip_gauge *const *g, const plqcd_hip_field
       int dev = 0, ncu = 0, per = 0;
       (void)hipGetDevice(&dev);
       (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-      if (mode == PLQCD_HOP_COMPAT) {
-         (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, hop_batch_kernel<PLQCD_HOP_COMPAT, 256, S, G>, 256, 0);
-         const int grid = std::max(1, std::min(total, std::max(per, 1) * ncu));
-         hipLaunchKernelGGL((hop_batch_kernel<PLQCD_HOP_COMPAT, 256, S, G>), dim3(grid), dim3(256), 0, st, d, n, total);
-      } else {
-         (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, hop_batch_kernel<PLQCD_HOP_WILSON, 256, S, G>, 256, 0);
-         const int grid = std::max(1, std::min(total, std::max(per, 1) * ncu));
-         hipLaunchKernelGGL((hop_batch_kernel<PLQCD_HOP_WILSON, 256, S, G>), dim3(grid), dim3(256), 0, st, d, n, total);
-      }
+      int nmarch = 0;
+      for (const BatchItem<S, G> &I : items) nmarch += I.march;
+      auto go = [&](auto kern) {
+         (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, 256, 0);
+         const int grid = std::max(8, std::min(total, std::max(per, 1) * ncu) / 8 * 8);
+         hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, st, d, n, total);
+      };
+      if (mode == PLQCD_HOP_COMPAT) go(hop_batch_kernel<PLQCD_HOP_COMPAT, 0, 256, S, G>);
+      else if (nmarch == 0) go(hop_batch_kernel<PLQCD_HOP_WILSON, 0, 256, S, G>);
+      else if (nmarch == n) go(hop_batch_kernel<PLQCD_HOP_WILSON, 1, 256, S, G>);
+      else go(hop_batch_kernel<PLQCD_HOP_WILSON, 2, 256, S, G>);
       e = hipGetLastError();
    }
    if (e != hipSuccess) rc = plqcd::set_error(10, "hop_batch: %s", hipGetErrorString(e));

@@ -17,9 +17,9 @@
 namespace plqcd {
 inline int reduce_chunks(int Vh) { return (Vh + REDUCE_CHUNK - 1) / REDUCE_CHUNK; }
 
-// out[r] = sum of partial[r][first .. first+n) for r < R (fixed order), on `st`.
-// The partials are overwritten.
-int reduce_partials(double *partial, size_t stride, int first, int n, int R, double *out, hipStream_t st);
+// out[r] = sum of partial[r][first .. first+n) for r0 <= r < r0+R (fixed
+// order), on `st`.  The partials are overwritten.
+int reduce_partials(double *partial, size_t stride, int first, int n, int R, double *out, hipStream_t st, int r0 = 0);
 }  // namespace plqcd
 
 // Stage 1 inside a block whose lanes are (js, r), tid = js * R + r, with

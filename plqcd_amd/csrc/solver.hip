@@ -57,6 +57,18 @@ __global__ void k_admit(const int *mask, const double *src, double *rr, double *
 
 namespace plqcd {
 
+// The smallest window of slots holding every slot with flag[s] != 0.
+Window window_of(const std::vector<int> &flag)
+{
+   int lo = -1, hi = -1;
+   for (int s = 0; s < (int)flag.size(); s++)
+      if (flag[s]) {
+         if (lo < 0) lo = s;
+         hi = s;
+      }
+   return lo < 0 ? Window{0, 0} : Window{lo, hi - lo + 1};
+}
+
 size_t CgEngine::bytes_needed(const plqcd_hip_lattice *lat, int sloppy, int S)
 {
    const size_t f64 = 2 * spinor_parity_bytes(PLQCD_PREC_F64, lat->Vh) * S;
@@ -90,6 +102,7 @@ int CgEngine::create(const plqcd_hip_gauge *g, const plqcd_hip_gauge *gs, int S,
    e->x_ = x;
    e->own_bx_ = b == nullptr;
    e->slot_.assign(S, CgSlot{});
+   e->graphs_.assign(S + 1, Graph{});
    e->active_.assign(S, 0);
    e->update_.assign(S, 0);
    if (int rc = e->alloc(gs->prec)) {
@@ -118,6 +131,8 @@ int CgEngine::alloc(int sloppy)
    PLQCD_HIP_TRY(hipMalloc(&scal_, 7 * S_ * sizeof(double)));
    PLQCD_HIP_TRY(hipMalloc(&imask_, 2 * S_ * sizeof(int)));
    PLQCD_HIP_TRY(hipHostMalloc(&host_, 2 * S_ * sizeof(double), hipHostMallocDefault));
+   PLQCD_HIP_TRY(hipHostMalloc(&hmask_, 2 * S_ * sizeof(int), hipHostMallocDefault));
+   PLQCD_HIP_TRY(hipEventCreateWithFlags(&ev_, hipEventDisableTiming));
    hipStream_t st = lat_->stream;
    std::vector<double> z(7 * S_, 0.0);
    PLQCD_HIP_TRY(hipMemcpyAsync(scal_, z.data(), z.size() * sizeof(double), hipMemcpyHostToDevice, st));
@@ -133,8 +148,10 @@ int CgEngine::alloc(int sloppy)
 CgEngine::~CgEngine()
 {
    if (lat_) (void)hipStreamSynchronize(lat_->stream);
-   if (graph_) (void)hipGraphExecDestroy(graph_);
-   if (graph_def_) (void)hipGraphDestroy(graph_def_);
+   for (Graph &gr : graphs_) {
+      if (gr.exec) (void)hipGraphExecDestroy(gr.exec);
+      if (gr.def) (void)hipGraphDestroy(gr.def);
+   }
    const bool alias = (t1_ == qs_);
    for (plqcd_hip_field *f : {rs_, ps_, qs_, xs_}) plqcd_hip_field_destroy(f);
    if (!alias) plqcd_hip_field_destroy(t1_);
@@ -147,6 +164,8 @@ CgEngine::~CgEngine()
    if (scal_) (void)hipFree(scal_);
    if (imask_) (void)hipFree(imask_);
    if (host_) (void)hipHostFree(host_);
+   if (hmask_) (void)hipHostFree(hmask_);
+   if (ev_) (void)hipEventDestroy(ev_);
 }
 
 int CgEngine::free_slot() const
@@ -179,12 +198,14 @@ int CgEngine::admit(const std::vector<int> &which, const double *kappa, const do
       sl.max_iter = max_iter[k];
    }
    PLQCD_HIP_TRY(hipMemcpyAsync(update_d(), mask.data(), S_ * sizeof(int), hipMemcpyHostToDevice, st));
+   const Window w = window_of(mask);
+   if (w.nr == 0) return 0;
    // x = xs = ps = 0 ; rs = b ; |b|^2 (in FP64, before rounding to the sloppy precision)
-   if (int rc = blas_zero(x_, update_d())) return rc;
-   if (int rc = blas_zero(xs_, update_d())) return rc;
-   if (int rc = blas_zero(ps_, update_d())) return rc;
-   if (int rc = blas_copy(rs_, b_, partial_, update_d())) return rc;
-   if (int rc = reduce_partials(partial_, 2 * nc, 0, 2 * nc, S_, tmp_d(), st)) return rc;
+   if (int rc = blas_zero(x_, update_d(), w)) return rc;
+   if (int rc = blas_zero(xs_, update_d(), w)) return rc;
+   if (int rc = blas_zero(ps_, update_d(), w)) return rc;
+   if (int rc = blas_copy(rs_, b_, partial_, update_d(), w)) return rc;
+   if (int rc = reduce_partials(partial_, 2 * nc, 0, 2 * nc, w.nr, tmp_d(), st, w.r0)) return rc;
    hipLaunchKernelGGL(k_admit, dim3(1), dim3(kMaxSlots), 0, st, update_d(), tmp_d(), rr_d(), rr_new_d(), S_);
    PLQCD_HIP_TRY(hipGetLastError());
    PLQCD_HIP_TRY(hipMemcpyAsync(host_ + S_, tmp_d(), S_ * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -209,28 +230,38 @@ int CgEngine::admit(const std::vector<int> &which, const double *kappa, const do
    return 0;
 }
 
-int CgEngine::admit_host(int s, const spinor *b_host, double kappa, double tol, int max_iter)
+int CgEngine::admit_host(const std::vector<int> &which, const std::vector<const spinor *> &b_host, double kappa,
+                         double tol, int max_iter)
 {
-   if (s < 0 || s >= S_) return set_error(1, "cg: bad slot %d", s);
-   if (int rc = field_upload_range(b_, s, b_host, 0, lat_->V)) return rc;
-   return admit(std::vector<int>{s}, &kappa, &tol, &max_iter);
+   if (which.size() != b_host.size()) return set_error(1, "cg: one source per admitted slot");
+   for (size_t k = 0; k < which.size(); k++) {
+      if (which[k] < 0 || which[k] >= S_) return set_error(1, "cg: bad slot %d", which[k]);
+      if (int rc = field_upload_range(b_, which[k], b_host[k], 0, lat_->V)) return rc;
+   }
+   const std::vector<double> kap(which.size(), kappa), tl(which.size(), tol);
+   const std::vector<int> mi(which.size(), max_iter);
+   return admit(which, kap.data(), tl.data(), mi.data());
 }
 
-// One CG iteration on the sloppy fields.
-int CgEngine::iteration()
+// One CG iteration on the sloppy fields, for the slots 0 .. nr-1 (every
+// active slot lies below nr; inactive ones inside get zero steps).
+int CgEngine::iteration(int nr)
 {
    hipStream_t st = lat_->stream;
    const int nc = reduce_chunks(lat_->Vh);
+   const Window w{0, nr};
    hipLaunchKernelGGL(k_beta, dim3(1), dim3(kMaxSlots), 0, st, rr_d(), rr_new_d(), active_d(), beta_d(), S_);
-   if (int rc = blas_xpay(rs_, beta_d(), ps_)) return rc;
+   if (int rc = blas_xpay(rs_, beta_d(), ps_, w)) return rc;
    DslashOp m;
+   m.nr = nr;
    m.kappa = kappa_d();
    m.partial = partial_;
    if (int rc = dslash(gs_, ps_, qs_, PLQCD_PARITY_ALL, PLQCD_HOP_WILSON, m)) return rc;
-   if (int rc = reduce_partials(partial_, 2 * nc, 0, 2 * nc, S_, pap_d(), st)) return rc;
+   if (int rc = reduce_partials(partial_, 2 * nc, 0, 2 * nc, nr, pap_d(), st)) return rc;
    hipLaunchKernelGGL(k_alpha, dim3(1), dim3(kMaxSlots), 0, st, rr_d(), pap_d(), active_d(), alpha_d(), S_);
    // Ap = M^+ q, fused with xs += alpha p, rs -= alpha Ap and |rs|^2
    DslashOp md;
+   md.nr = nr;
    md.kappa = kappa_d();
    md.dag = 1;
    md.partial = partial_;
@@ -238,17 +269,17 @@ int CgEngine::iteration()
    md.fuse_p = ps_;
    md.fuse_x = xs_;
    if (int rc = dslash(gs_, qs_, rs_, PLQCD_PARITY_ALL, PLQCD_HOP_WILSON, md)) return rc;
-   if (int rc = reduce_partials(partial_, 2 * nc, 0, 2 * nc, S_, rr_new_d(), st)) return rc;
+   if (int rc = reduce_partials(partial_, 2 * nc, 0, 2 * nc, nr, rr_new_d(), st)) return rc;
    PLQCD_HIP_TRY(hipMemcpyAsync(host_, rr_new_d(), S_ * sizeof(double), hipMemcpyDeviceToHost, st));
    PLQCD_HIP_TRY(hipGetLastError());
    return 0;
 }
 
-int CgEngine::capture()
+int CgEngine::capture(int nr)
 {
    hipStream_t st = lat_->stream;
    PLQCD_HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-   const int rc = iteration();
+   const int rc = iteration(nr);
    hipGraph_t gdef = nullptr;
    const hipError_t e = hipStreamEndCapture(st, &gdef);
    if (rc) {
@@ -256,45 +287,79 @@ int CgEngine::capture()
       return rc;
    }
    if (e != hipSuccess) return set_error(10, "cg: graph capture failed: %s", hipGetErrorString(e));
-   graph_def_ = gdef;
-   PLQCD_HIP_TRY(hipGraphInstantiate(&graph_, gdef, nullptr, nullptr, 0));
+   Graph &gr = graphs_[nr];
+   gr.def = gdef;
+   PLQCD_HIP_TRY(hipGraphInstantiate(&gr.exec, gdef, nullptr, nullptr, 0));
    return 0;
+}
+
+int CgEngine::capture_all()
+{
+   if (!use_graph_) return 0;
+   for (int nr = 1; nr <= S_; nr++)
+      if (!graphs_[nr].exec)
+         if (int rc = capture(nr)) return rc;
+   return 0;
+}
+
+void CgEngine::abort_active(std::vector<int> &finished)
+{
+   for (int s = 0; s < S_; s++) {
+      CgSlot &sl = slot_[s];
+      if (!sl.active) continue;
+      sl.active = false;
+      sl.done = true;
+      sl.stats.converged = 0;
+      active_[s] = 0;
+      update_[s] = 0;
+      finished.push_back(s);
+   }
 }
 
 int CgEngine::launch()
 {
+   // slots 0 .. nr-1 hold every active slot (admission fills the lowest free slot)
+   const Window act = window_of(active_);
+   const int nr = act.r0 + act.nr;
+   if (nr == 0) return 0;
    if (use_graph_) {
-      if (!graph_)
-         if (int rc = capture()) return rc;
-      PLQCD_HIP_TRY(hipGraphLaunch(graph_, lat_->stream));
-      return 0;
+      if (!graphs_[nr].exec)
+         if (int rc = capture(nr)) return rc;
+      PLQCD_HIP_TRY(hipGraphLaunch(graphs_[nr].exec, lat_->stream));
+   } else {
+      if (int rc = iteration(nr)) return rc;
    }
-   return iteration();
+   PLQCD_HIP_TRY(hipEventRecord(ev_, lat_->stream));
+   return 0;
 }
 
-// x += xs ; r = b - M^+M x (into t2) ; rs = r ; tmp <- |r|^2, for the slots in update.
+// x += xs ; r = b - M^+M x (into t2) ; rs = r ; tmp <- |r|^2, for the slots in
+// update; the FP64 operator runs only over the window spanning them.
 int CgEngine::reliable_update()
 {
    hipStream_t st = lat_->stream;
    const int nc = reduce_chunks(lat_->Vh);
-   if (int rc = blas_accum(x_, xs_, 1, update_d())) return rc;
+   const Window w = window_of(update_);
+   if (w.nr == 0) return 0;
+   if (int rc = blas_accum(x_, xs_, 1, update_d(), w)) return rc;
    DslashOp m;
+   m.r0 = w.r0;
+   m.nr = w.nr;
    m.kappa = kappa_d();
    if (int rc = dslash(g_, x_, t1_, PLQCD_PARITY_ALL, PLQCD_HOP_WILSON, m)) return rc;
    m.dag = 1;
    if (int rc = dslash(g_, t1_, t2_, PLQCD_PARITY_ALL, PLQCD_HOP_WILSON, m)) return rc;
-   if (int rc = blas_residual(b_, t2_, t2_, partial_, update_d())) return rc;
-   if (int rc = reduce_partials(partial_, 2 * nc, 0, 2 * nc, S_, tmp_d(), st)) return rc;
-   if (int rc = blas_copy(rs_, t2_, nullptr, update_d())) return rc;
+   if (int rc = blas_residual(b_, t2_, t2_, partial_, update_d(), w)) return rc;
+   if (int rc = reduce_partials(partial_, 2 * nc, 0, 2 * nc, w.nr, tmp_d(), st, w.r0)) return rc;
+   if (int rc = blas_copy(rs_, t2_, nullptr, update_d(), w)) return rc;
    PLQCD_HIP_TRY(hipMemcpyAsync(host_ + S_, tmp_d(), S_ * sizeof(double), hipMemcpyDeviceToHost, st));
-   PLQCD_HIP_TRY(hipStreamSynchronize(st));
    return 0;
 }
 
-int CgEngine::finish(std::vector<int> &finished)
+int CgEngine::begin_finish(bool &updating)
 {
    hipStream_t st = lat_->stream;
-   PLQCD_HIP_TRY(hipStreamSynchronize(st));
+   updating = false;
    int nupd = 0;
    for (int s = 0; s < S_; s++) {
       update_[s] = 0;
@@ -309,10 +374,21 @@ int CgEngine::finish(std::vector<int> &finished)
       }
    }
    if (!nupd) return 0;
-   PLQCD_HIP_TRY(hipMemcpyAsync(update_d(), update_.data(), S_ * sizeof(int), hipMemcpyHostToDevice, st));
+   std::copy(update_.begin(), update_.end(), hmask_ + S_);
+   PLQCD_HIP_TRY(hipMemcpyAsync(update_d(), hmask_ + S_, S_ * sizeof(int), hipMemcpyHostToDevice, st));
    if (int rc = reliable_update()) return rc;
+   PLQCD_HIP_TRY(hipEventRecord(ev_, st));
+   updating = true;
+   return 0;
+}
+
+int CgEngine::end_finish(std::vector<int> &finished)
+{
+   hipStream_t st = lat_->stream;
+   bool any = false;
    for (int s = 0; s < S_; s++) {
       if (!update_[s]) continue;
+      any = true;
       CgSlot &sl = slot_[s];
       const double rt = host_[S_ + s];
       sl.stats.reliable_updates++;
@@ -326,9 +402,31 @@ int CgEngine::finish(std::vector<int> &finished)
          active_[s] = 0;
          finished.push_back(s);
       }
+      update_[s] = 0;
    }
+   if (!any) return 0;
+   // read by the next iteration, which the stream orders after these copies;
+   // the host rewrites host_/hmask_ only after that iteration has completed
+   std::copy(active_.begin(), active_.end(), hmask_);
    PLQCD_HIP_TRY(hipMemcpyAsync(rr_new_d(), host_, S_ * sizeof(double), hipMemcpyHostToDevice, st));
-   PLQCD_HIP_TRY(hipMemcpyAsync(active_d(), active_.data(), S_ * sizeof(int), hipMemcpyHostToDevice, st));
+   PLQCD_HIP_TRY(hipMemcpyAsync(active_d(), hmask_, S_ * sizeof(int), hipMemcpyHostToDevice, st));
+   return 0;
+}
+
+bool CgEngine::ready() const
+{
+   const hipError_t e = hipEventQuery(ev_);
+   return e != hipErrorNotReady;  // errors surface at the next synchronizing call
+}
+
+int CgEngine::finish(std::vector<int> &finished)
+{
+   hipStream_t st = lat_->stream;
+   PLQCD_HIP_TRY(hipStreamSynchronize(st));
+   bool updating = false;
+   if (int rc = begin_finish(updating)) return rc;
+   if (updating) PLQCD_HIP_TRY(hipStreamSynchronize(st));
+   if (int rc = end_finish(finished)) return rc;
    PLQCD_HIP_TRY(hipStreamSynchronize(st));
    return 0;
 }

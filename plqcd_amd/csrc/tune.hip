@@ -52,7 +52,7 @@ extern "C" int plqcd_hip_autotune(const plqcd_hip_gauge *g, const plqcd_hip_fiel
    HopTuning prev{};
    const bool had = plqcd::lookup_tuning(key, &prev);
    double best_t = 1e300;
-   HopTuning best_c = plqcd::default_tuning(g->prec, in->nrhs);
+   HopTuning best_c = plqcd::default_tuning(g->prec, g->links, in->nrhs);
    hipEvent_t e0, e1;
    PLQCD_HIP_TRY(hipEventCreate(&e0));
    PLQCD_HIP_TRY(hipEventCreate(&e1));

@@ -295,3 +295,15 @@ class Field:
 def hop(gauge: Gauge, fin: Field, fout: Field, parity: int = PARITY_ALL, mode: int = HOP_WILSON) -> None:
     """fout = H fin on `parity` (asynchronous on the lattice stream)."""
     check(lib().plqcd_hip_hop(gauge.h, fin.h, fout.h, parity, mode), "hop")
+
+
+def hop_batch(gauges, fins, fouts, mode: int = HOP_WILSON) -> None:
+    """fouts[k] = H fins[k] (full lattice) for independent lattices in one
+    persistent launch; all items share one storage format and one stream."""
+    n = len(gauges)
+    if not (n == len(fins) == len(fouts)) or n == 0:
+        raise ValueError("hop_batch: gauges, fins and fouts must be non-empty and of equal length")
+    garr = (C.c_void_p * n)(*[g.h.value for g in gauges])
+    iarr = (C.c_void_p * n)(*[f.h.value for f in fins])
+    oarr = (C.c_void_p * n)(*[f.h.value for f in fouts])
+    check(lib().plqcd_hip_hop_batch(n, garr, iarr, oarr, mode), "hop_batch")

@@ -16,8 +16,9 @@ from .hopping import PREC_F16, PREC_F32, PREC_F64  # noqa: F401
 
 class Server:
     def __init__(self, sloppy_prec: int = PREC_F16, sloppy_links: int = 12, slots: int = 12,
-                 hbm_budget: int = 0, use_graph: bool = True, device: int = 0):
-        cfg = capi.ServeConfig(device, sloppy_prec, sloppy_links, slots, hbm_budget, int(use_graph))
+                 hbm_budget: int = 0, use_graph: bool = True, device: int = 0, transfer_threads: int = 0):
+        cfg = capi.ServeConfig(device, sloppy_prec, sloppy_links, slots, hbm_budget, int(use_graph),
+                               transfer_threads)
         h = C.c_void_p()
         check(lib().plqcd_hip_server_create(C.byref(cfg), C.byref(h)), "server_create")
         self.h = h
@@ -28,7 +29,9 @@ class Server:
         """b: complex128 array (nrhs, V, 4, 3).  Returns the request id."""
         V = int(np.prod(L))
         b = np.ascontiguousarray(b, dtype=np.complex128).reshape(-1, V, 4, 3)
-        x = np.zeros_like(b)
+        # the server writes every element; empty_like commits no host pages
+        # until the solution arrives (zeros_like would touch them all now)
+        x = np.empty_like(b)
         req = capi.ServeRequest()
         req.L = (C.c_int * 4)(*L)
         req.gauge_seed = gauge_seed
@@ -65,9 +68,11 @@ class Server:
         return done
 
     def info(self) -> dict:
-        v = (C.c_double * 4)()
+        v = (C.c_double * 8)()
         check(lib().plqcd_hip_server_info(self.h, v), "server_info")
-        return {"budget": v[0], "used": v[1], "entries": int(v[2]), "failed": int(v[3])}
+        return {"budget": v[0], "used": v[1], "entries": int(v[2]), "failed": int(v[3]),
+                "engines_created": int(v[4]), "evictions": int(v[5]), "transfer_bytes": v[6],
+                "transfer_seconds": v[7]}
 
     def close(self):
         if self.h:

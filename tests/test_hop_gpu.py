@@ -351,3 +351,63 @@ def test_tuning_rejects_bad_tiles(hp):
     assert lib.plqcd_hip_hop_tuning(0, 1, 1, 256, 4, 1, -1, 2) != 0
     assert lib.plqcd_hip_hop_tuning(0, 1, 1, 256, 4, 3, 0, 0) != 0
     assert lib.plqcd_hip_hop_tuning_reset(-1) == 0
+
+
+# ------------------------------------------------------------ ragged batch
+
+RAGGED = [[((4, 4, 4, 8), 3), ((8, 4, 6, 6), 1), ((8, 8, 8, 16), 4), ((4, 4, 4, 4), 12)],  # mixed kinds
+          [((4, 4, 4, 16), 3), ((8, 8, 8, 32), 1), ((4, 4, 8, 64), 2)]]                    # t-march only (F64/F32)
+
+
+@pytest.mark.parametrize("shapes", RAGGED)
+@pytest.mark.parametrize("prec,links,tol", [(0, 18, TOL64), (1, 12, TOL32), (2, 12, TOL16)])
+def test_ragged_batch_matches_separate_and_oracle(hp, orc, shapes, prec, links, tol):
+    """One persistent launch over lattices of different volumes and RHS counts
+    (t-march items and items whose time extent takes gather blocks, mixed or
+    all of one kind) gives each item bitwise its own launch's output, which
+    matches the oracle."""
+    from plqcd_amd import capi
+    lib = capi.lib()
+    lats, gs, fis, fos, fsep, inputs_ = [], [], [], [], [], []
+    for k, (L, R) in enumerate(shapes):
+        V = int(np.prod(L))
+        lat = hp.Lattice(L)
+        if lats:
+            lat.set_stream(lib.plqcd_hip_lattice_stream(lats[0].h))
+        u = hp.random_gauge_su3(V, 300 + k)
+        qs = [hp.random_spinor(V, 400 + 13 * k + r) for r in range(R)]
+        g = hp.Gauge(lat, prec, links).upload(u)
+        fi, fo, fs = hp.Field(lat, prec, R), hp.Field(lat, prec, R), hp.Field(lat, prec, R)
+        for r, q in enumerate(qs):
+            fi.upload(q, r)
+        lats.append(lat); gs.append(g); fis.append(fi); fos.append(fo); fsep.append(fs); inputs_.append((L, u, qs))
+    try:
+        for mode, omode in ((hp.HOP_WILSON, orc.MODE_WILSON), (hp.HOP_COMPAT, orc.MODE_COMPAT)):
+            hp.hop_batch(gs, fis, fos, mode)
+            for k in range(len(shapes)):
+                hp.hop(gs[k], fis[k], fsep[k], hp.PARITY_ALL, mode)
+            lats[0].sync()
+            for k, (L, u, qs) in enumerate(inputs_):
+                for r, q in enumerate(qs):
+                    out = fos[k].download(r)
+                    assert np.array_equal(out, fsep[k].download(r)), (mode, L, r)
+                    assert site_rel_err(out, orc.hopping_full(q, u, L, omode)) < tol, (mode, L, r)
+    finally:
+        for objs in (fis, fos, fsep, gs):
+            for o in objs:
+                o.close()
+        for lat in reversed(lats):
+            lat.close()
+
+
+def test_ragged_batch_errors(hp):
+    lat1, lat2 = hp.Lattice((4, 4, 4, 4)), hp.Lattice((4, 4, 4, 8))
+    g1, g2 = hp.Gauge(lat1), hp.Gauge(lat2, 1, 12)
+    f1, o1 = hp.Field(lat1), hp.Field(lat1)
+    f2, o2 = hp.Field(lat2, 1), hp.Field(lat2, 1)
+    with pytest.raises(RuntimeError):
+        hp.hop_batch([g1, g2], [f1, f2], [o1, o2])  # mixed storage formats
+    with pytest.raises(RuntimeError):
+        hp.hop_batch([g1], [f1], [f1])  # in == out
+    for o in (f1, o1, f2, o2, g1, g2, lat2, lat1):
+        o.close()
