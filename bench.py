@@ -41,7 +41,7 @@ os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
 METRIC = "hopping GFLOP/s & %HBM roofline at 32^3x64; batched mixed vs FP64 solves/sec @1e-10"
 HBM_PEAK_GBS = 8000.0
 FLOP_PER_SITE = 1320
-BYTES_PER_SITE = {(0, 18): 960, (0, 12): 768, (1, 18): 480, (1, 12): 384, (2, 12): 200}
+BYTES_PER_SITE = {(0, 18): 960, (0, 12): 768, (1, 18): 480, (1, 12): 384, (2, 12): 200, (2, 18): 248}
 PREC_NAMES = {0: "f64", 1: "f32", 2: "f16"}
 
 
@@ -119,11 +119,11 @@ def cpu_baseline(L, seconds: float):
 
 def formats_run(L, mode, steps=200, warmup=10):
     """The same application of H in the other storage formats (FP64 with 12-real
-    links, FP32 with 18- and 12-real links, FP16 with 12-real links), default
-    launch shapes; per format: time per launch, GFLOP/s and compulsory GB/s."""
+    links, FP32 and FP16 with 18- and 12-real links), default launch shapes;
+    per format: time per launch, GFLOP/s and compulsory GB/s."""
     V = int(np.prod(L))
     out = {}
-    for prec, links in ((0, 12), (1, 18), (1, 12), (2, 12)):
+    for prec, links in ((0, 12), (1, 18), (1, 12), (2, 18), (2, 12)):
         ms = gpu_run(L, prec, links, steps, warmup, mode)
         t = ms / 1e3 / steps
         gbs = BYTES_PER_SITE[(prec, links)] * V / t / 1e9

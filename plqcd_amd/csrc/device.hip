@@ -75,6 +75,22 @@ __global__ void pack_gauge(const d2 *__restrict__ aos, void *dst, int prec, int 
       f4 *o = static_cast<f4 *>(dst) + lk * 3 * Vh + j;
       for (int k = 0; k < 3; k++)
          o[(size_t)k * Vh] = f4{(float)u[2 * k].x, (float)u[2 * k].y, (float)u[2 * k + 1].x, (float)u[2 * k + 1].y};
+   } else if (links == 18) {  // F16: u4 (k 0..3), u4 (k 4..7), then u32 (k 8)
+      u4 *a = static_cast<u4 *>(dst) + lk * Vh + j;
+      u4 *b = static_cast<u4 *>(dst) + (8 + lk) * Vh + j;
+      uint32_t *c = reinterpret_cast<uint32_t *>(static_cast<u4 *>(dst) + (size_t)16 * Vh) + lk * Vh + j;
+      u4 wa, wb;
+      wa.x = f2_to_h2((float)u[0].x, (float)u[0].y);
+      wa.y = f2_to_h2((float)u[1].x, (float)u[1].y);
+      wa.z = f2_to_h2((float)u[2].x, (float)u[2].y);
+      wa.w = f2_to_h2((float)u[3].x, (float)u[3].y);
+      wb.x = f2_to_h2((float)u[4].x, (float)u[4].y);
+      wb.y = f2_to_h2((float)u[5].x, (float)u[5].y);
+      wb.z = f2_to_h2((float)u[6].x, (float)u[6].y);
+      wb.w = f2_to_h2((float)u[7].x, (float)u[7].y);
+      *a = wa;
+      *b = wb;
+      *c = f2_to_h2((float)u[8].x, (float)u[8].y);
    } else {
       u4 *a = static_cast<u4 *>(dst) + lk * Vh + j;
       u2 *b = reinterpret_cast<u2 *>(static_cast<u4 *>(dst) + (size_t)8 * Vh) + lk * Vh + j;
@@ -194,7 +210,8 @@ extern "C" int plqcd_hip_gauge_create(plqcd_hip_lattice *lat, int prec, int link
    if (!lat || !out) return plqcd::set_error(1, "plqcd_hip_gauge_create: null argument");
    *out = nullptr;
    const bool ok = (prec == PLQCD_PREC_F64 && (links == 18 || links == 12)) ||
-                   (prec == PLQCD_PREC_F32 && (links == 18 || links == 12)) || (prec == PLQCD_PREC_F16 && links == 12);
+                   (prec == PLQCD_PREC_F32 && (links == 18 || links == 12)) ||
+                   (prec == PLQCD_PREC_F16 && (links == 18 || links == 12));
    if (!ok) return plqcd::set_error(2, "plqcd_hip_gauge_create: unsupported precision %d with %d-real links", prec, links);
    plqcd_hip_gauge *g = new (std::nothrow) plqcd_hip_gauge();
    if (!g) return plqcd::set_error(4, "out of host memory");

@@ -12,8 +12,10 @@
 //               + [parity][Vh][rhs] float max-abs scale     (  4 B/site)
 //   gauge  F64  [parity][mu][9 or 6][Vh]          double2
 //   gauge  F32  [parity][mu][9][Vh] float2  or  [parity][mu][3][Vh] float4
-//   gauge  F16  [parity][mu][Vh] 8 x fp16 (row 0 + row1.c0) and
-//               [parity][mu][Vh] 4 x fp16 (row1.c1, row1.c2)
+//   gauge  F16  12-real: [parity][mu][Vh] 8 x fp16 (row 0 + row1.c0) and
+//               [parity][mu][Vh] 4 x fp16 (row1.c1, row1.c2);
+//               18-real: [parity][mu][Vh] 8 x fp16 (k 0..3), [parity][mu][Vh]
+//               8 x fp16 (k 4..7) and [parity][mu][Vh] 2 x fp16 (k 8)
 // Component order inside a spinor: c = 3*spin + colour.  Inside a link:
 // k = 3*row + col.  With 12-real links row 2 is rebuilt as conj(row0 x row1).
 #pragma once
@@ -75,7 +77,7 @@ inline size_t gauge_bytes(int prec, int links, int Vh)
    switch (prec) {
       case PLQCD_PREC_F64: return n * links / 2 * 16;
       case PLQCD_PREC_F32: return n * links * 4;
-      default: return n * (16 + 8);
+      default: return n * (links == 18 ? 16 + 16 + 4 : 16 + 8);
    }
 }
 

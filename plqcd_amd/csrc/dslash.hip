@@ -11,22 +11,28 @@
 // format: the LDS-tiled t-march for F64 (tiles of 32 x 4 x 1 sites, 32
 // t-slices: 434 us against 514 us for the untiled t-march) and F32 (32 x 4 x 4
 // with 12-real links: 174 against 217 us; 32 x 4 x 2 with 18-real links, whose
-// 512-lane tile spills registers), the gather kernel for F16 (129 us, the tile
-// 136 us).  Shapes that do not fit a lattice fall back to the untiled t-march
-// or the gather kernel.  Several RHS per launch, per precision: R lanes per site
-// favour the gather kernel for 16-bit storage and XCDs sweeping the lattice
-// together.
+// 512-lane tile spills registers), the untiled t-march over 4 t-slices for F16
+// (102 us with 12-real links, the gather kernel 116 us).  Shapes that do not
+// fit a lattice fall back to the untiled t-march or the gather kernel.  Several
+// RHS per launch, per precision: XCDs sweeping the lattice together, with
+// short t-marches (F16 at 12 RHS: 82 us per RHS, the gather kernel 87 us).
 namespace {
 struct FormatShape {
    int prec, links;
    HopTuning t;
 };
-const FormatShape kOneRhs[5] = {{PLQCD_PREC_F64, 18, {1, 1, 256, 32, 1, 4, 1}},
+const FormatShape kOneRhs[6] = {{PLQCD_PREC_F64, 18, {1, 1, 256, 32, 1, 4, 1}},
                                 {PLQCD_PREC_F64, 12, {1, 1, 256, 32, 1, 4, 1}},
                                 {PLQCD_PREC_F32, 18, {2, 8, 256, 8, 1, 4, 2}},
                                 {PLQCD_PREC_F32, 12, {2, 8, 256, 8, 1, 4, 4}},
-                                {PLQCD_PREC_F16, 12, {1, 1, 128, 0, 1, 0, 0}}};
-const HopTuning kMultiRhs[3] = {{2, 8, 256, 16, 1}, {2, 8, 256, 4, 1}, {2, 8, 256, 0, 1}};
+                                {PLQCD_PREC_F16, 12, {1, 1, 256, 4, 1, 0, 0}},
+                                {PLQCD_PREC_F16, 18, {2, 8, 512, 4, 1, 0, 0}}};
+const HopTuning kMultiRhs[3] = {{2, 8, 256, 16, 1, 0, 0}, {2, 8, 256, 4, 1, 0, 0}, {2, 8, 256, 4, 1, 0, 0}};
+// Several RHS with the solver epilogues (M, and M^+ with the CG update): the
+// F16 gather kernel, whose 64-72 VGPRs leave room for more waves to overlap
+// the epilogue's streams (12-RHS FP16 CG at 32^3 x 64: 0.264 s, with the
+// t-march 0.306 s).
+const HopTuning kMultiRhsSolver[3] = {{2, 8, 256, 16, 1, 0, 0}, {2, 8, 256, 4, 1, 0, 0}, {2, 8, 256, 0, 1, 0, 0}};
 bool g_tuning_set[3] = {false, false, false};  // set by plqcd_hip_hop_tuning
 
 HopTuning one_rhs_shape(int prec, int links)
@@ -44,10 +50,11 @@ HopTuning one_rhs_shape(int prec, int links)
 HopTuning g_tuning[3] = {one_rhs_shape(PLQCD_PREC_F64, 18), one_rhs_shape(PLQCD_PREC_F32, 18),
                          one_rhs_shape(PLQCD_PREC_F16, 12)};
 
-HopTuning plqcd::default_tuning(int prec, int links, int R)
+HopTuning plqcd::default_tuning(int prec, int links, int R, bool solver)
 {
    if (g_tuning_set[prec]) return g_tuning[prec];
-   return R <= 1 ? one_rhs_shape(prec, links) : kMultiRhs[prec];
+   if (R <= 1) return one_rhs_shape(prec, links);
+   return solver ? kMultiRhsSolver[prec] : kMultiRhs[prec];
 }
 
 namespace {
@@ -97,6 +104,7 @@ extern "C" int plqcd_hip_hop_batch(int n, const plqcd_hip_gauge *const *g, const
       case PLQCD_PREC_F32 * 100 + 18: return plqcd::hop_batch_launch<PLQCD_PREC_F32, 18>(n, g, in, out, mode);
       case PLQCD_PREC_F32 * 100 + 12: return plqcd::hop_batch_launch<PLQCD_PREC_F32, 12>(n, g, in, out, mode);
       case PLQCD_PREC_F16 * 100 + 12: return plqcd::hop_batch_launch<PLQCD_PREC_F16, 12>(n, g, in, out, mode);
+      case PLQCD_PREC_F16 * 100 + 18: return plqcd::hop_batch_launch<PLQCD_PREC_F16, 18>(n, g, in, out, mode);
       default: return plqcd::set_error(1, "hop_batch: unsupported gauge format");
    }
 }
@@ -144,6 +152,7 @@ int plqcd::dslash(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip
       case PLQCD_PREC_F32 * 100 + 18: return plqcd::hop_launch<PLQCD_PREC_F32, 18>(g, in, out, parity, mode, op);
       case PLQCD_PREC_F32 * 100 + 12: return plqcd::hop_launch<PLQCD_PREC_F32, 12>(g, in, out, parity, mode, op);
       case PLQCD_PREC_F16 * 100 + 12: return plqcd::hop_launch<PLQCD_PREC_F16, 12>(g, in, out, parity, mode, op);
+      case PLQCD_PREC_F16 * 100 + 18: return plqcd::hop_launch<PLQCD_PREC_F16, 18>(g, in, out, parity, mode, op);
       default: return plqcd::set_error(1, "hop: unsupported gauge format");
    }
 }

@@ -169,6 +169,188 @@ __device__ __forceinline__ void recon(C acc[12], const C a[6])
    else recon_plus<MU>(acc, a);
 }
 
+// ---------------------------------------------------------------- terms
+//
+// term_u:    r = U (1 + SG gamma_mu) f        (a forward term, before recon)
+// term_udag: r = U^+ (1 + SG gamma_mu) f      (a backward term)
+// accum:     acc += reconstruction of r
+// on the held forms of fields.h.  F64 and F32 project, multiply and add in
+// the working precision.
+
+template <int MU, int SG, class C>
+__device__ __forceinline__ void term_u(const SpinV<C> &f, const LinkV<C> &u, HalfV<C> &r)
+{
+   C h[6];
+   proj<MU, SG>(f.e, h);
+   mul_u(u.e, h, r.e);
+}
+template <int MU, int SG, class C>
+__device__ __forceinline__ void term_udag(const SpinV<C> &f, const LinkV<C> &u, HalfV<C> &r)
+{
+   C h[6];
+   proj<MU, SG>(f.e, h);
+   mul_udag(u.e, h, r.e);
+}
+template <int MU, int SG, bool LOWER, class C>
+__device__ __forceinline__ void accum(C acc[12], const HalfV<C> &r)
+{
+   recon<MU, SG, LOWER>(acc, r.e);
+}
+
+// F16: the projection adds stored half pairs of one site (one v_pk_add_f16
+// per half-spinor component; sign changes and the factor i are operand
+// modifiers), the SU(3) multiply is v_dot2_f32_f16 on half pairs with float
+// accumulation, and the site scale of the neighbour multiplies the float
+// result as it is accumulated (acc += s * r, one fma per real).  Every kernel
+// variant uses these functions, so a site's result does not depend on the
+// variant.
+
+// X g for X = 0: +1, 1: -1, 2: +i, 3: -i
+template <int X>
+__device__ __forceinline__ hh2 xmul(hh2 g)
+{
+   if (X == 0) return g;
+   if (X == 1) return -g;
+   if (X == 2) return hh2{-g.y, g.x};
+   return hh2{g.y, -g.x};
+}
+
+// The projection of proj_minus / proj_plus as h[c] = f[c] + X0 f[P0 + c],
+// h[3 + c] = f[3 + c] + X1 f[P1 + c]; the lower rows of the reconstruction
+// (recon_minus / recon_plus) as acc[6 + c] += Y2 r[S2 + c], acc[9 + c] +=
+// Y3 r[S3 + c] (factors coded as in xmul).
+template <int MU, int SG>
+struct SpinTab {
+   static constexpr int P0 = (MU == 0 || MU == 3) ? 6 : 9;
+   static constexpr int P1 = 15 - P0;
+   static constexpr int X0 = ((MU == 0 || MU == 2) ? 0 : 2) ^ (SG > 0);
+   static constexpr int X1 = (MU == 0 ? 0 : MU == 1 ? 2 : MU == 2 ? 1 : 3) ^ (SG > 0);
+   static constexpr int S2 = (MU == 0 || MU == 3) ? 0 : 3;
+   static constexpr int S3 = 3 - S2;
+   static constexpr int Y2 = (MU == 0 ? 0 : MU == 1 ? 3 : MU == 2 ? 1 : 3) ^ (SG > 0);
+   static constexpr int Y3 = (MU == 0 ? 0 : MU == 1 ? 3 : MU == 2 ? 0 : 2) ^ (SG > 0);
+};
+
+// h = upper((1 + SG gamma_mu) f) in halves, and its swapped pairs (h.im, h.re)
+template <int MU, int SG>
+__device__ __forceinline__ void proj_h(const SpinH &f, hh2 h[6], hh2 hs[6])
+{
+   typedef SpinTab<MU, SG> T;
+#pragma unroll
+   for (int c = 0; c < 3; c++) {
+      h[c] = f.h(c) + xmul<T::X0>(f.h(T::P0 + c));
+      h[3 + c] = f.h(3 + c) + xmul<T::X1>(f.h(T::P1 + c));
+      hs[c] = hswap(h[c]);
+      hs[3 + c] = hswap(h[3 + c]);
+   }
+}
+
+// One colour vector (h, hs = hswap(h)) times U (DAG false) or U^+ (DAG
+// true), r = (re, im) of the 3 result colours, as one assembly block of 18
+// v_dot2_f32_f16 (see fields.h for why assembly and the closing s_nop):
+//   U h:   re_a = sum_b U_ab . (h_b.re, -h_b.im)   im_a = sum_b U_ab . hs_b
+//   U^+ h: re_a = sum_b U_ba . h_b                  im_a = sum_b U_ba . (hs_b.re, -hs_b.im)
+template <bool DAG>
+__device__ __forceinline__ void mat_h(const LinkH &u, const hh2 h[3], const hh2 hs[3], float r[6])
+{
+   // operands: %0..%5 re0 im0 re1 im1 re2 im2, %6..%14 u_k (k = 3 row + col),
+   // %15..%17 h, %18..%20 hs
+   if constexpr (!DAG) {
+      asm(PLQCD_D2Z("%0", "%6", "%15", PLQCD_NHI)
+          PLQCD_D2Z("%1", "%6", "%18", "")
+          PLQCD_D2Z("%2", "%9", "%15", PLQCD_NHI)
+          PLQCD_D2Z("%3", "%9", "%18", "")
+          PLQCD_D2Z("%4", "%12", "%15", PLQCD_NHI)
+          PLQCD_D2Z("%5", "%12", "%18", "")
+          PLQCD_D2("%0", "%7", "%16", PLQCD_NHI)
+          PLQCD_D2("%1", "%7", "%19", "")
+          PLQCD_D2("%2", "%10", "%16", PLQCD_NHI)
+          PLQCD_D2("%3", "%10", "%19", "")
+          PLQCD_D2("%4", "%13", "%16", PLQCD_NHI)
+          PLQCD_D2("%5", "%13", "%19", "")
+          PLQCD_D2("%0", "%8", "%17", PLQCD_NHI)
+          PLQCD_D2("%1", "%8", "%20", "")
+          PLQCD_D2("%2", "%11", "%17", PLQCD_NHI)
+          PLQCD_D2("%3", "%11", "%20", "")
+          PLQCD_D2("%4", "%14", "%17", PLQCD_NHI)
+          PLQCD_D2("%5", "%14", "%20", "")
+          "s_nop 2"
+          : "=&v"(r[0]), "=&v"(r[1]), "=&v"(r[2]), "=&v"(r[3]), "=&v"(r[4]), "=&v"(r[5])
+          : "v"(u.h(0)), "v"(u.h(1)), "v"(u.h(2)), "v"(u.h(3)), "v"(u.h(4)), "v"(u.h(5)), "v"(u.h(6)),
+            "v"(u.h(7)), "v"(u.h(8)), "v"(h[0]), "v"(h[1]), "v"(h[2]), "v"(hs[0]), "v"(hs[1]), "v"(hs[2]));
+   } else {
+      asm(PLQCD_D2Z("%0", "%6", "%15", "")
+          PLQCD_D2Z("%1", "%6", "%18", PLQCD_NHI)
+          PLQCD_D2Z("%2", "%7", "%15", "")
+          PLQCD_D2Z("%3", "%7", "%18", PLQCD_NHI)
+          PLQCD_D2Z("%4", "%8", "%15", "")
+          PLQCD_D2Z("%5", "%8", "%18", PLQCD_NHI)
+          PLQCD_D2("%0", "%9", "%16", "")
+          PLQCD_D2("%1", "%9", "%19", PLQCD_NHI)
+          PLQCD_D2("%2", "%10", "%16", "")
+          PLQCD_D2("%3", "%10", "%19", PLQCD_NHI)
+          PLQCD_D2("%4", "%11", "%16", "")
+          PLQCD_D2("%5", "%11", "%19", PLQCD_NHI)
+          PLQCD_D2("%0", "%12", "%17", "")
+          PLQCD_D2("%1", "%12", "%20", PLQCD_NHI)
+          PLQCD_D2("%2", "%13", "%17", "")
+          PLQCD_D2("%3", "%13", "%20", PLQCD_NHI)
+          PLQCD_D2("%4", "%14", "%17", "")
+          PLQCD_D2("%5", "%14", "%20", PLQCD_NHI)
+          "s_nop 2"
+          : "=&v"(r[0]), "=&v"(r[1]), "=&v"(r[2]), "=&v"(r[3]), "=&v"(r[4]), "=&v"(r[5])
+          : "v"(u.h(0)), "v"(u.h(1)), "v"(u.h(2)), "v"(u.h(3)), "v"(u.h(4)), "v"(u.h(5)), "v"(u.h(6)),
+            "v"(u.h(7)), "v"(u.h(8)), "v"(h[0]), "v"(h[1]), "v"(h[2]), "v"(hs[0]), "v"(hs[1]), "v"(hs[2]));
+   }
+}
+
+template <int MU, int SG>
+__device__ __forceinline__ void term_u(const SpinH &f, const LinkH &u, HalfH &r)
+{
+   hh2 h[6], hs[6];
+   proj_h<MU, SG>(f, h, hs);
+   mat_h<false>(u, h, hs, r.e);
+   mat_h<false>(u, h + 3, hs + 3, r.e + 6);
+   r.e[12] = f.s();
+}
+
+template <int MU, int SG>
+__device__ __forceinline__ void term_udag(const SpinH &f, const LinkH &u, HalfH &r)
+{
+   hh2 h[6], hs[6];
+   proj_h<MU, SG>(f, h, hs);
+   mat_h<true>(u, h, hs, r.e);
+   mat_h<true>(u, h + 3, hs + 3, r.e + 6);
+   r.e[12] = f.s();
+}
+
+// acc += s * Y (x + i y), Y coded as in xmul
+template <int Y>
+__device__ __forceinline__ void acc_add(f2 &acc, float s, float x, float y)
+{
+   if (Y == 0) { acc.x = fma(s, x, acc.x);  acc.y = fma(s, y, acc.y); }
+   if (Y == 1) { acc.x = fma(-s, x, acc.x); acc.y = fma(-s, y, acc.y); }
+   if (Y == 2) { acc.x = fma(-s, y, acc.x); acc.y = fma(s, x, acc.y); }
+   if (Y == 3) { acc.x = fma(s, y, acc.x);  acc.y = fma(-s, x, acc.y); }
+}
+
+// recon_minus (SG < 0; lower rows only if LOWER) / recon_plus (SG > 0)
+template <int MU, int SG, bool LOWER>
+__device__ __forceinline__ void accum(f2 acc[12], const HalfH &r)
+{
+   typedef SpinTab<MU, SG> T;
+   const float s = r.e[12];
+#pragma unroll
+   for (int c = 0; c < 3; c++) {
+      acc_add<0>(acc[c], s, r.e[2 * c], r.e[2 * c + 1]);
+      acc_add<0>(acc[3 + c], s, r.e[2 * (3 + c)], r.e[2 * (3 + c) + 1]);
+      if (SG > 0 || LOWER) {
+         acc_add<T::Y2>(acc[6 + c], s, r.e[2 * (T::S2 + c)], r.e[2 * (T::S2 + c) + 1]);
+         acc_add<T::Y3>(acc[9 + c], s, r.e[2 * (T::S3 + c)], r.e[2 * (T::S3 + c) + 1]);
+      }
+   }
+}
+
 // Forward term of direction MU for output site (p, j); jnb = site of x+mu.
 // H uses (1 - gamma_mu) forward and (1 + gamma_mu) backward; the adjoint
 // H^+ = gamma5 H gamma5 has the signs swapped (gamma5 anticommutes with
@@ -176,28 +358,28 @@ __device__ __forceinline__ void recon(C acc[12], const C a[6])
 template <int MU, int MODE, bool DAG, class SIO, class GIO>
 __device__ __forceinline__ void hop_fwd(const SIO &in, const GIO &g, int p, int j, int jnb, typename SIO::C acc[12])
 {
-   typedef typename SIO::C C;
    constexpr int SG = DAG ? 1 : -1;
-   C f[12], h[6], u[9], r[6];
-   in.load(1 - p, jnb, f);
-   proj<MU, SG>(f, h);
-   g.load(MODE == MODE_COMPAT ? 0 : p, MU, j, u);
-   mul_u(u, h, r);
-   recon<MU, SG, MODE != MODE_COMPAT>(acc, r);
+   typename SIO::Sp f;
+   typename GIO::U u;
+   typename SIO::Hv r;
+   in.load_sp(1 - p, jnb, f);
+   g.load_u(MODE == MODE_COMPAT ? 0 : p, MU, j, u);
+   term_u<MU, SG>(f, u, r);
+   accum<MU, SG, MODE != MODE_COMPAT>(acc, r);
 }
 
 // Backward term of direction MU; jnb = site of x-mu (its link U_mu(x-mu) too).
 template <int MU, bool DAG, class SIO, class GIO>
 __device__ __forceinline__ void hop_bwd(const SIO &in, const GIO &g, int p, int jnb, typename SIO::C acc[12])
 {
-   typedef typename SIO::C C;
    constexpr int SG = DAG ? -1 : 1;
-   C f[12], k[6], u[9], r[6];
-   in.load(1 - p, jnb, f);
-   proj<MU, SG>(f, k);
-   g.load(1 - p, MU, jnb, u);
-   mul_udag(u, k, r);
-   recon<MU, SG, true>(acc, r);
+   typename SIO::Sp f;
+   typename GIO::U u;
+   typename SIO::Hv r;
+   in.load_sp(1 - p, jnb, f);
+   g.load_u(1 - p, MU, jnb, u);
+   term_udag<MU, SG>(f, u, r);
+   accum<MU, SG, true>(acc, r);
 }
 
 template <int MU, int MODE, bool DAG, class SIO, class GIO>
@@ -360,24 +542,24 @@ __device__ __forceinline__ void hop_block_rb(const SIO &in, const SIO &out, cons
       const SIO si = in.at_rhs(a.r0 + gi * RB);
 #define PLQCD_RB_DIR(MU)                                                                      \
    {                                                                                          \
-      C u[9];                                                                                 \
-      g.load(MODE == MODE_COMPAT ? 0 : p, MU, j, u);                                          \
+      typename GIO::U u;                                                                      \
+      g.load_u(MODE == MODE_COMPAT ? 0 : p, MU, j, u);                                        \
       _Pragma("unroll") for (int k = 0; k < RB; k++)                                          \
       {                                                                                       \
-         C f[12], h[6], r[6];                                                                 \
-         si.at_rhs(a.r0 + gi * RB + k).load(1 - p, n.up[MU], f);                              \
-         proj<MU, SF>(f, h);                                                                  \
-         mul_u(u, h, r);                                                                      \
-         recon<MU, SF, MODE != MODE_COMPAT>(acc[k], r);                                       \
+         typename SIO::Sp f;                                                                  \
+         typename SIO::Hv r;                                                                  \
+         si.at_rhs(a.r0 + gi * RB + k).load_sp(1 - p, n.up[MU], f);                           \
+         term_u<MU, SF>(f, u, r);                                                             \
+         accum<MU, SF, MODE != MODE_COMPAT>(acc[k], r);                                       \
       }                                                                                       \
-      g.load(1 - p, MU, n.dn[MU], u);                                                         \
+      g.load_u(1 - p, MU, n.dn[MU], u);                                                       \
       _Pragma("unroll") for (int k = 0; k < RB; k++)                                          \
       {                                                                                       \
-         C f[12], h[6], r[6];                                                                 \
-         si.at_rhs(a.r0 + gi * RB + k).load(1 - p, n.dn[MU], f);                              \
-         proj<MU, SB>(f, h);                                                                  \
-         mul_udag(u, h, r);                                                                   \
-         recon<MU, SB, true>(acc[k], r);                                                      \
+         typename SIO::Sp f;                                                                  \
+         typename SIO::Hv r;                                                                  \
+         si.at_rhs(a.r0 + gi * RB + k).load_sp(1 - p, n.dn[MU], f);                           \
+         term_udag<MU, SB>(f, u, r);                                                          \
+         accum<MU, SB, true>(acc[k], r);                                                      \
       }                                                                                       \
    }
       PLQCD_RB_DIR(0)
@@ -472,15 +654,16 @@ __device__ __forceinline__ void hop_tmarch_block(const SIO &in, const SIO &out, 
    }
    const int Lt = a.L[3];
    // prologue: cen = psi(t0); b = U_t(t0-1)^+ (1 -+ g_t) psi(t0-1)
-   C cen[12], b[6];
+   typename SIO::Sp cen;
+   typename SIO::Hv b;
    if (valid) {
-      C f[12], u[9], kk[6];
-      si.load(q, c + t0 * S, cen);
+      typename SIO::Sp f;
+      typename GIO::U u;
+      si.load_sp(q, c + t0 * S, cen);
       const int tm = (t0 == 0 ? Lt : t0) - 1;
-      si.load(q ^ 1, c + tm * S, f);
-      proj<3, -SF>(f, kk);
-      g.load(q ^ 1, 3, c + tm * S, u);
-      mul_udag(u, kk, b);
+      si.load_sp(q ^ 1, c + tm * S, f);
+      g.load_u(q ^ 1, 3, c + tm * S, u);
+      term_udag<3, -SF>(f, u, b);
    }
    for (int s = 0; s < a.T; s++) {
       const int t = t0 + s;
@@ -498,22 +681,26 @@ __device__ __forceinline__ void hop_tmarch_block(const SIO &in, const SIO &out, 
          hop_bwd<1, DAG>(si, g, p, cd[1] + base, acc);
          hop_fwd<2, PLQCD_HOP_WILSON, DAG>(si, g, p, j, cu[2] + base, acc);
          hop_bwd<2, DAG>(si, g, p, cd[2] + base, acc);
-         C f[12];
+         typename SIO::Sp f;
          {
-            C h[6], u[9], rr[6], kk[6];
+            typename GIO::U u;
+            typename SIO::Hv rr;
             const int tp = (t + 1 == Lt) ? 0 : t + 1;
-            si.load(p ^ 1, c + tp * S, f);
-            proj<3, SF>(f, h);
-            g.load(p, 3, j, u);
-            mul_u(u, h, rr);
-            recon<3, SF, true>(acc, rr);
-            recon<3, -SF, true>(acc, b);
-            proj<3, -SF>(cen, kk);
-            mul_udag(u, kk, b);
+            si.load_sp(p ^ 1, c + tp * S, f);
+            g.load_u(p, 3, j, u);
+            term_u<3, SF>(f, u, rr);
+            accum<3, SF, true>(acc, rr);
+            accum<3, -SF, true>(acc, b);
+            term_udag<3, -SF>(cen, u, b);
          }
-         nrm = epilogue<EPI, NORM>(si, so, a, a.r0 + r, p, j, acc, cen, fuse);
-#pragma unroll
-         for (int m = 0; m < 12; m++) cen[m] = f[m];
+         if (EPI >= 1) {
+            C cv[12];
+            SIO::to_c(cen, cv);
+            nrm = epilogue<EPI, NORM>(si, so, a, a.r0 + r, p, j, acc, cv, fuse);
+         } else {
+            nrm = epilogue<EPI, NORM>(si, so, a, a.r0 + r, p, j, acc, (const C *)nullptr, fuse);
+         }
+         cen = f;
       }
       if (NORM) chunk_reduce<BLOCK>(nrm, js, r, a.R, p * a.nchunk + (sb * a.spb + js + base) / REDUCE_CHUNK, valid,
                                     a.partial + (size_t)(a.r0 + r) * 2 * a.nchunk);
@@ -562,32 +749,32 @@ __global__ __launch_bounds__(BLOCK) void hop_batch_kernel(const BatchItem<SIO, G
 // LDS slots su / sd (>= 0), the others from memory; same operation sequence
 // as hop_fwd + hop_bwd.
 template <int MU, bool DAG, int BLOCK, class SIO, class GIO>
-__device__ __forceinline__ void tile_dir(const SIO &in, const GIO &g, const typename SIO::C (*buf)[BLOCK], int p,
+__device__ __forceinline__ void tile_dir(const SIO &in, const GIO &g, const typename SIO::Sp::E (*buf)[BLOCK], int p,
                                          int j, int base, int cu, int cd, int su, int sd, typename SIO::C acc[12])
 {
-   typedef typename SIO::C C;
+   typedef typename SIO::Sp Sp;
    constexpr int SG = DAG ? 1 : -1;
-   C f[12], h[6], u[9], r[6];
+   Sp f;
+   typename GIO::U u;
+   typename SIO::Hv r;
    if (su >= 0) {
 #pragma unroll
-      for (int m = 0; m < 12; m++) f[m] = buf[m][su];
+      for (int m = 0; m < Sp::N; m++) f.e[m] = buf[m][su];
    } else {
-      in.load(1 - p, cu + base, f);
+      in.load_sp(1 - p, cu + base, f);
    }
-   proj<MU, SG>(f, h);
-   g.load(p, MU, j, u);
-   mul_u(u, h, r);
-   recon<MU, SG, true>(acc, r);
+   g.load_u(p, MU, j, u);
+   term_u<MU, SG>(f, u, r);
+   accum<MU, SG, true>(acc, r);
    if (sd >= 0) {
 #pragma unroll
-      for (int m = 0; m < 12; m++) f[m] = buf[m][sd];
+      for (int m = 0; m < Sp::N; m++) f.e[m] = buf[m][sd];
    } else {
-      in.load(1 - p, cd + base, f);
+      in.load_sp(1 - p, cd + base, f);
    }
-   proj<MU, -SG>(f, h);
-   g.load(1 - p, MU, cd + base, u);
-   mul_udag(u, h, r);
-   recon<MU, -SG, true>(acc, r);
+   g.load_u(1 - p, MU, cd + base, u);
+   term_udag<MU, -SG>(f, u, r);
+   accum<MU, -SG, true>(acc, r);
 }
 
 // LDS-tiled t-march (wilson mode, full lattice, one RHS).  A block owns the
@@ -608,8 +795,10 @@ __global__ __launch_bounds__(BLOCK) void hop_tile_kernel(SIO in, SIO out, GIO g,
    in = in.at_rhs(a.r0);  // the launch's one RHS
    out = out.at_rhs(a.r0);
    constexpr int SF = DAG ? 1 : -1;  // forward projector sign
-   __shared__ C sm[2][12][BLOCK];
-   __shared__ C sb[6][BLOCK];  // the lane's backward t term b (read and written by that lane only)
+   typedef typename SIO::Sp Sp;
+   typedef typename SIO::Hv Hv;
+   __shared__ typename Sp::E sm[2][Sp::N][BLOCK];
+   __shared__ typename Hv::E sb[Hv::N][BLOCK];  // the lane's backward t term b (read and written by that lane only)
    const int S = a.S;
    constexpr int ncol = BLOCK / 2;
    const int nty = a.L[1] / a.ty;
@@ -655,17 +844,18 @@ __global__ __launch_bounds__(BLOCK) void hop_tile_kernel(SIO in, SIO out, GIO g,
    const int Lt = a.L[3];
    // prologue: psi(t0) into the first buffer; b = U_t(t0-1)^+ (1 -+ g_t) psi(t0-1)
    {
-      C f[12], u[9], kk[6], b[6];
-      in.load(q, c + t0 * S, f);
+      Sp f;
+      typename GIO::U u;
+      Hv b;
+      in.load_sp(q, c + t0 * S, f);
 #pragma unroll
-      for (int m = 0; m < 12; m++) sm[0][m][tid] = f[m];
+      for (int m = 0; m < Sp::N; m++) sm[0][m][tid] = f.e[m];
       const int tm = (t0 == 0 ? Lt : t0) - 1;
-      in.load(q ^ 1, c + tm * S, f);
-      proj<3, -SF>(f, kk);
-      g.load(q ^ 1, 3, c + tm * S, u);
-      mul_udag(u, kk, b);
+      in.load_sp(q ^ 1, c + tm * S, f);
+      g.load_u(q ^ 1, 3, c + tm * S, u);
+      term_udag<3, -SF>(f, u, b);
 #pragma unroll
-      for (int m = 0; m < 6; m++) sb[m][tid] = b[m];
+      for (int m = 0; m < Hv::N; m++) sb[m][tid] = b.e[m];
    }
    for (int s = 0; s < a.T; s++) {
       const int t = t0 + s;
@@ -674,8 +864,8 @@ __global__ __launch_bounds__(BLOCK) void hop_tile_kernel(SIO in, SIO out, GIO g,
       const int j = c + base;
       // buf holds the centre spinors of this t-slice; the t+1 spinors go to
       // nxt, whose readers (the previous slice) have all passed this barrier
-      C(*buf)[BLOCK] = sm[s & 1];
-      C(*nxt)[BLOCK] = sm[(s + 1) & 1];
+      typename Sp::E(*buf)[BLOCK] = sm[s & 1];
+      typename Sp::E(*nxt)[BLOCK] = sm[(s + 1) & 1];
       __syncthreads();
       C acc[12];
 #pragma unroll
@@ -689,30 +879,32 @@ __global__ __launch_bounds__(BLOCK) void hop_tile_kernel(SIO in, SIO out, GIO g,
       tile_dir<2, DAG, BLOCK>(in, g, buf, p, j, base, cu[2], cd[2], su[2], sd[2], acc);
       __builtin_amdgcn_sched_barrier(0);
       {
-         C f[12], h[6], u[9], rr[6], kk[6], b[6];
+         Sp f;
+         typename GIO::U u;
+         Hv rr, b;
          const int tp = (t + 1 == Lt) ? 0 : t + 1;
-         in.load(p ^ 1, c + tp * S, f);
+         in.load_sp(p ^ 1, c + tp * S, f);
 #pragma unroll
-         for (int m = 0; m < 12; m++) nxt[m][tid] = f[m];
-         proj<3, SF>(f, h);
-         g.load(p, 3, j, u);
-         mul_u(u, h, rr);
-         recon<3, SF, true>(acc, rr);
+         for (int m = 0; m < Sp::N; m++) nxt[m][tid] = f.e[m];
+         g.load_u(p, 3, j, u);
+         term_u<3, SF>(f, u, rr);
+         accum<3, SF, true>(acc, rr);
 #pragma unroll
-         for (int m = 0; m < 6; m++) b[m] = sb[m][tid];
-         recon<3, -SF, true>(acc, b);
+         for (int m = 0; m < Hv::N; m++) b.e[m] = sb[m][tid];
+         accum<3, -SF, true>(acc, b);
 #pragma unroll
-         for (int m = 0; m < 12; m++) f[m] = buf[m][tid];
-         proj<3, -SF>(f, kk);
-         mul_udag(u, kk, b);
+         for (int m = 0; m < Sp::N; m++) f.e[m] = buf[m][tid];
+         term_udag<3, -SF>(f, u, b);
 #pragma unroll
-         for (int m = 0; m < 6; m++) sb[m][tid] = b[m];
+         for (int m = 0; m < Hv::N; m++) sb[m][tid] = b.e[m];
       }
       double nrm;
       if (EPI >= 1) {
+         Sp cs;
          C cen[12];
 #pragma unroll
-         for (int m = 0; m < 12; m++) cen[m] = buf[m][tid];
+         for (int m = 0; m < Sp::N; m++) cs.e[m] = buf[m][tid];
+         SIO::to_c(cs, cen);
          nrm = epilogue<EPI, NORM>(in, out, a, a.r0, p, j, acc, cen, fuse);
       } else {
          nrm = epilogue<EPI, NORM>(in, out, a, a.r0, p, j, acc, (const C *)nullptr, fuse);
@@ -735,7 +927,8 @@ void launch_one(int variant, dim3 grid, dim3 block, hipStream_t st, S si, S so, 
          hipLaunchKernelGGL((hop_tmarch_kernel<DAG, EPI, NORM, BLOCK, S, G>), grid, block, 0, st, si, so, gg, a, f);
    } else if (variant == V_TILE) {
       // two LDS buffers of BLOCK spinors and BLOCK half-spinors must fit the CU's 160 KiB
-      if constexpr (MODE == PLQCD_HOP_WILSON && (2 * 12 + 6) * BLOCK * sizeof(typename S::C) <= 136 * 1024)
+      if constexpr (MODE == PLQCD_HOP_WILSON &&
+                    (2 * sizeof(typename S::Sp) + sizeof(typename S::Hv)) * BLOCK <= 136 * 1024)
          hipLaunchKernelGGL((hop_tile_kernel<DAG, EPI, NORM, BLOCK, S, G>), grid, block, 0, st, si, so, gg, a, f);
    } else if (variant == V_RB2) {
       hipLaunchKernelGGL((hop_rb_kernel<MODE, DAG, EPI, NORM, BLOCK, 2, S, G>), grid, block, 0, st, si, so, gg, a, f);
@@ -768,7 +961,7 @@ int launch_t(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip_fiel
    // lanes: the window of right-hand sides [op.r0, op.r0 + R) of the fields
    const int R = op.nr > 0 ? op.nr : in->nrhs;
    a.r0 = op.nr > 0 ? op.r0 : 0;
-   HopTuning tu = plqcd::default_tuning(PREC, LINKS, R);
+   HopTuning tu = plqcd::default_tuning(PREC, LINKS, R, op.kappa != nullptr);
    plqcd::lookup_tuning(plqcd::TuneKey{{lat->L[0], lat->L[1], lat->L[2], lat->L[3]}, PREC, LINKS, R}, &tu);
    // RHS per lane (RHS-blocked gather) when it divides R
    const int rb = (tu.rb == 2 || tu.rb == 4) && R % tu.rb == 0 ? tu.rb : 1;

@@ -622,7 +622,7 @@ extern "C" int plqcd_hip_server_create(const plqcd_serve_config *cfg, plqcd_hip_
       return plqcd::set_error(2, "server: transfer_threads must be 0..64");
    const bool ok = cfg->sloppy_prec == PLQCD_PREC_F64 ||
                    (cfg->sloppy_prec == PLQCD_PREC_F32 && (cfg->sloppy_links == 18 || cfg->sloppy_links == 12)) ||
-                   (cfg->sloppy_prec == PLQCD_PREC_F16 && cfg->sloppy_links == 12);
+                   (cfg->sloppy_prec == PLQCD_PREC_F16 && (cfg->sloppy_links == 18 || cfg->sloppy_links == 12));
    if (!ok) return plqcd::set_error(2, "server: unsupported sloppy precision");
    plqcd_hip_server *s = new (std::nothrow) plqcd_hip_server();
    if (!s) return plqcd::set_error(4, "out of host memory");

@@ -38,7 +38,7 @@ def test_apply_m_matches_oracle(hp, dagger):
         assert site_rel_err(out.download(r), ref) < 1e-13
 
 
-@pytest.mark.parametrize("sloppy", [None, (1, 12), (1, 18), (2, 12)])
+@pytest.mark.parametrize("sloppy", [None, (1, 12), (1, 18), (2, 12), (2, 18)])
 def test_cg_true_residual(hp, sloppy):
     from oracle import solver as osv
     from plqcd_amd import solver as sv
