@@ -215,7 +215,7 @@ def solver_suite():
     V = int(np.prod(L))
     u = hp.random_gauge_su3(V, 1)
     lat = hp.Lattice(L)
-    for prec, links in ((0, 18), (1, 12), (2, 12)):
+    for prec, links in ((0, 18), (1, 12), (2, 18), (2, 12)):
         g = hp.Gauge(lat, prec, links).upload(u)
         for R in (1, 4, 12, 24):
             fi, fo = hp.Field(lat, prec, R), hp.Field(lat, prec, R)
@@ -303,7 +303,7 @@ def calib_suite():
     u = hp.random_gauge_su3(V, 1)
     q = hp.random_spinor(V, 2)
     lat = hp.Lattice(L)
-    for prec, links in ((0, 18), (1, 18), (1, 12), (2, 12)):
+    for prec, links in ((0, 18), (1, 18), (1, 12), (2, 18), (2, 12)):
         g = hp.Gauge(lat, prec, links).upload(u)
         fi, fo = hp.Field(lat, prec).upload(q), hp.Field(lat, prec)
         for _ in range(2):

@@ -174,6 +174,9 @@ int plqcd_hip_lattice_sync(plqcd_hip_lattice *lat);
 int plqcd_hip_timer_start(plqcd_hip_lattice *lat);
 int plqcd_hip_timer_stop(plqcd_hip_lattice *lat, double *ms);
 
+/* A gauge field in storage precision `prec` (PLQCD_PREC_F64/F32/F16) with
+ * 18-real links, or 12-real links whose row 2 the kernels rebuild as
+ * conj(row0 x row1) (exact for SU(3) links up to rounding). */
 int plqcd_hip_gauge_create(plqcd_hip_lattice *lat, int prec, int links, plqcd_hip_gauge **out);
 void plqcd_hip_gauge_destroy(plqcd_hip_gauge *g);
 /* u: 4V links in the reference layout u[4*i+mu], even-odd site order.
@@ -254,7 +257,7 @@ typedef struct plqcd_hip_server plqcd_hip_server;
 typedef struct {
    int device;         /* HIP device                                            */
    int sloppy_prec;    /* inner CG precision: PLQCD_PREC_F64 / F32 / F16         */
-   int sloppy_links;   /* 18 or 12 (F16: 12)                                     */
+   int sloppy_links;   /* 18 or 12                                               */
    int slots;          /* right-hand sides per engine, 1..32                     */
    size_t hbm_budget;  /* bytes the server may allocate (0: free HBM - 8 GiB)    */
    int use_graph;      /* replay captured CG iterations                          */
@@ -339,8 +342,8 @@ int plqcd_hip_autotune_clear(void);
 
 /* One application of H (full lattice, given mode) on each of n independent
  * lattices -- each with its own gauge field and fields, any volume, up to 32
- * RHS -- in a single persistent launch.  All items share storage precision,
- * link format and stream.  Asynchronous. */
+ * RHS -- in one persistent launch per 16 lattices.  All items share storage
+ * precision, link format and stream.  Asynchronous. */
 int plqcd_hip_hop_batch(int n, const plqcd_hip_gauge *const *g, const plqcd_hip_field *const *in,
                         plqcd_hip_field *const *out, int mode);
 

@@ -602,6 +602,13 @@ struct BatchItem {
    int nb;     // blocks of this item
    int march;  // t-march blocks (a.T t-slices per lane) instead of gather blocks
 };
+// A launch takes up to kBatchItems items by value, as a kernel argument (no
+// device copy of the list); longer batches take several launches.
+constexpr int kBatchItems = 16;
+template <class SIO, class GIO>
+struct BatchList {
+   BatchItem<SIO, GIO> it[kBatchItems];
+};
 
 // t-march variant (wilson mode, full lattice): a thread owns one (x,y,z)
 // column and walks T consecutive t-slices.  Along t each spinor and each U_t
@@ -721,7 +728,7 @@ __global__ __launch_bounds__(BLOCK) void hop_tmarch_kernel(SIO in, SIO out, GIO 
 // every item t-march blocks, 2: each item its own kind (the kernel then holds
 // both code paths and needs the registers of the larger).
 template <int MODE, int KIND, int BLOCK, class SIO, class GIO>
-__global__ __launch_bounds__(BLOCK) void hop_batch_kernel(const BatchItem<SIO, GIO> *items, int nitems, int total)
+__global__ __launch_bounds__(BLOCK) void hop_batch_kernel(const BatchList<SIO, GIO> items, int nitems, int total)
 {
    constexpr int kGroup = 8;
    const int xcd = blockIdx.x & 7, per = gridDim.x >> 3;
@@ -730,8 +737,8 @@ __global__ __launch_bounds__(BLOCK) void hop_batch_kernel(const BatchItem<SIO, G
       const int b = ((i / kGroup) * 8 + xcd) * kGroup + i % kGroup;
       if (b >= total) break;
       int it = 0;
-      while (it + 1 < nitems && b >= items[it + 1].first) it++;
-      const BatchItem<SIO, GIO> &I = items[it];
+      while (it + 1 < nitems && b >= items.it[it + 1].first) it++;
+      const BatchItem<SIO, GIO> &I = items.it[it];
       if constexpr (MODE == PLQCD_HOP_WILSON && KIND == 1) {
          hop_tmarch_block<false, 0, false, BLOCK>(I.in, I.out, I.g, I.a, b - I.first, CgFuse<SIO>{});
       } else if constexpr (MODE == PLQCD_HOP_WILSON && KIND == 2) {
@@ -1050,53 +1057,59 @@ int launch_batch_t(int n, const plqcd_hip_gauge *const *g, const plqcd_hip_field
 {
    typedef SpinorIO<PREC> S;
    typedef GaugeIO<PREC, LINKS> G;
-   std::vector<BatchItem<S, G>> items(n);
-   int total = 0, maxR = 1;
-   for (int i = 0; i < n; i++) maxR = std::max(maxR, in[i]->nrhs);
-   if (maxR * REDUCE_CHUNK > 256) return plqcd::set_error(2, "hop_batch: at most 32 right-hand sides per item");
-   for (int i = 0; i < n; i++) {
-      BatchItem<S, G> &I = items[i];
-      I.in = spinor_io<PREC>(in[i]);
-      I.out = spinor_io<PREC>(out[i]);
-      I.g = gauge_io<PREC, LINKS>(g[i]);
-      const plqcd_hip_lattice *lat = g[i]->lat;
-      I.a = make_hop_args(lat, PLQCD_PARITY_ALL);
-      I.a.R = in[i]->nrhs;
-      I.a.spb = REDUCE_CHUNK * (256 / (REDUCE_CHUNK * I.a.R));
-      // wilson items march along t with the precision's t-slice count when it
-      // divides the time extent; the others use gather blocks
-      const int T = plqcd::default_tuning(PREC, LINKS, I.a.R).tmarch;
-      I.march = mode == PLQCD_HOP_WILSON && T > 0 && lat->L[3] % T == 0;
-      I.a.T = I.march ? T : 1;
-      I.nb = I.march ? 2 * ((I.a.S + I.a.spb - 1) / I.a.spb) * (lat->L[3] / T) : 2 * ((I.a.Vh + I.a.spb - 1) / I.a.spb);
-      I.first = total;
-      total += I.nb;
-   }
+   static_assert(sizeof(BatchList<S, G>) <= 4096, "kernel argument too large");
+   for (int i = 0; i < n; i++)
+      if (in[i]->nrhs * REDUCE_CHUNK > 256) return plqcd::set_error(2, "hop_batch: at most 32 right-hand sides per item");
    hipStream_t st = g[0]->lat->stream;
-   BatchItem<S, G> *d = nullptr;
-   PLQCD_HIP_TRY(hipMallocAsync((void **)&d, n * sizeof(BatchItem<S, G>), st));
-   hipError_t e = hipMemcpyAsync(d, items.data(), n * sizeof(BatchItem<S, G>), hipMemcpyHostToDevice, st);
-   int rc = 0;
-   if (e == hipSuccess) {
-      int dev = 0, ncu = 0, per = 0;
-      (void)hipGetDevice(&dev);
-      (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-      int nmarch = 0;
-      for (const BatchItem<S, G> &I : items) nmarch += I.march;
+   int dev = 0, ncu = 0;
+   (void)hipGetDevice(&dev);
+   (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+   for (int i0 = 0; i0 < n; i0 += kBatchItems) {
+      const int m = std::min(kBatchItems, n - i0);
+      // block size: the largest whole number of reduction chunks of sites
+      // times the RHS count that fits 256 lanes, maximised over the items (a
+      // batch of 12-RHS items runs 192-lane blocks with no idle lane)
+      int block = 64;
+      for (int i = 0; i < m; i++) {
+         const int R = in[i0 + i]->nrhs;
+         block = std::max(block, REDUCE_CHUNK * (256 / (REDUCE_CHUNK * R)) * R);
+      }
+      BatchList<S, G> L{};
+      int total = 0, nmarch = 0;
+      for (int i = 0; i < m; i++) {
+         BatchItem<S, G> &I = L.it[i];
+         I.in = spinor_io<PREC>(in[i0 + i]);
+         I.out = spinor_io<PREC>(out[i0 + i]);
+         I.g = gauge_io<PREC, LINKS>(g[i0 + i]);
+         const plqcd_hip_lattice *lat = g[i0 + i]->lat;
+         I.a = make_hop_args(lat, PLQCD_PARITY_ALL);
+         I.a.R = in[i0 + i]->nrhs;
+         I.a.spb = REDUCE_CHUNK * (block / (REDUCE_CHUNK * I.a.R));
+         // wilson items march along t with the precision's t-slice count when it
+         // divides the time extent; the others use gather blocks
+         const int T = plqcd::default_tuning(PREC, LINKS, I.a.R).tmarch;
+         I.march = mode == PLQCD_HOP_WILSON && T > 0 && lat->L[3] % T == 0;
+         I.a.T = I.march ? T : 1;
+         I.nb = I.march ? 2 * ((I.a.S + I.a.spb - 1) / I.a.spb) * (lat->L[3] / T)
+                        : 2 * ((I.a.Vh + I.a.spb - 1) / I.a.spb);
+         I.first = total;
+         total += I.nb;
+         nmarch += I.march;
+      }
       auto go = [&](auto kern) {
-         (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, 256, 0);
+         int per = 0;
+         (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, block, 0);
          const int grid = std::max(8, std::min(total, std::max(per, 1) * ncu) / 8 * 8);
-         hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, st, d, n, total);
+         hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, st, L, m, total);
       };
       if (mode == PLQCD_HOP_COMPAT) go(hop_batch_kernel<PLQCD_HOP_COMPAT, 0, 256, S, G>);
       else if (nmarch == 0) go(hop_batch_kernel<PLQCD_HOP_WILSON, 0, 256, S, G>);
-      else if (nmarch == n) go(hop_batch_kernel<PLQCD_HOP_WILSON, 1, 256, S, G>);
+      else if (nmarch == m) go(hop_batch_kernel<PLQCD_HOP_WILSON, 1, 256, S, G>);
       else go(hop_batch_kernel<PLQCD_HOP_WILSON, 2, 256, S, G>);
-      e = hipGetLastError();
+      const hipError_t e = hipGetLastError();
+      if (e != hipSuccess) return plqcd::set_error(10, "hop_batch: %s", hipGetErrorString(e));
    }
-   if (e != hipSuccess) rc = plqcd::set_error(10, "hop_batch: %s", hipGetErrorString(e));
-   (void)hipFreeAsync(d, st);
-   return rc;
+   return 0;
 }
 
 }  // namespace

@@ -356,7 +356,9 @@ def test_tuning_rejects_bad_tiles(hp):
 # ------------------------------------------------------------ ragged batch
 
 RAGGED = [[((4, 4, 4, 8), 3), ((8, 4, 6, 6), 1), ((8, 8, 8, 16), 4), ((4, 4, 4, 4), 12)],  # mixed kinds
-          [((4, 4, 4, 16), 3), ((8, 8, 8, 32), 1), ((4, 4, 8, 64), 2)]]                    # t-march only (F64/F32)
+          [((4, 4, 4, 16), 3), ((8, 8, 8, 32), 1), ((4, 4, 8, 64), 2)],                    # t-march only (F64/F32)
+          [((4, 4, 4, 8), 12), ((8, 4, 4, 8), 12)],                                        # 192-lane blocks
+          [((4, 4, 4, 4 * (1 + k % 2)), 1 + k % 3) for k in range(18)]]                    # two launches of <= 16
 
 
 @pytest.mark.parametrize("shapes", RAGGED)
