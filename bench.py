@@ -135,9 +135,10 @@ def formats_run(L, mode, steps=200, warmup=10):
     return out
 
 
-def solve_compare(L, nrhs=12, sloppy=(2, 12), kappa=0.124, tol=1e-10):
+def solve_compare(L, nrhs=12, sloppy=(2, 12), kappa=0.124, tol=1e-10, reps=3):
     """Plain FP64 CG on one RHS vs the mixed-precision CG on a batch of nrhs,
-    same gauge field and sources, true residual <= tol.  Returns a dict."""
+    same gauge field and sources, true residual <= tol; each solve timed
+    `reps` times after one untimed solve, the median reported.  Returns a dict."""
     from plqcd_amd import hopping as hp, solver as sv
     V = int(np.prod(L))
     u = hp.random_gauge_su3(V, 1)
@@ -151,11 +152,15 @@ def solve_compare(L, nrhs=12, sloppy=(2, 12), kappa=0.124, tol=1e-10):
         for r in range(R):
             b.upload(hp.random_spinor(V, 100 + r), r)
         sv.cg_solve(g, b, x, kappa, tol, sloppy=sl)  # graph capture and first-touch costs
-        lat.sync()
-        t0 = time.perf_counter()
-        st = sv.cg_solve(g, b, x, kappa, tol, sloppy=sl)
-        t = time.perf_counter() - t0
-        out[label] = {"rhs": R, "seconds": round(t, 4), "solves_per_s": round(R / t, 3),
+        ts = []
+        for _ in range(reps):
+            lat.sync()
+            t0 = time.perf_counter()
+            st = sv.cg_solve(g, b, x, kappa, tol, sloppy=sl)
+            ts.append(time.perf_counter() - t0)
+        t = float(np.median(ts))
+        out[label] = {"rhs": R, "seconds": round(t, 4), "seconds_min_max": [round(min(ts), 4), round(max(ts), 4)],
+                      "solves_per_s": round(R / t, 3),
                       "iterations": max(s_["iterations"] for s_ in st),
                       "reliable_updates": max(s_["reliable_updates"] for s_ in st),
                       "max_true_residual": max(s_["true_residual"] for s_ in st),

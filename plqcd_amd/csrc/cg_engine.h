@@ -91,7 +91,7 @@ class CgEngine {
    bool use_graph_ = true;
    bool own_bx_ = false;
    plqcd_hip_field *b_ = nullptr, *x_ = nullptr;
-   plqcd_hip_field *t1_ = nullptr, *t2_ = nullptr;                                          // F64
+   plqcd_hip_field *t1_ = nullptr;                                                  // F64
    plqcd_hip_field *rs_ = nullptr, *ps_ = nullptr, *qs_ = nullptr, *xs_ = nullptr;  // sloppy
    double *partial_ = nullptr;
    double *scal_ = nullptr;  // kappa, rr, rr_new, pap, alpha, beta, tmp (7 x S)

@@ -133,7 +133,7 @@ int plqcd::dslash(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip
    if (in == out) return plqcd::set_error(1, "hop: in and out must be distinct fields");
    if (in->lat != g->lat || out->lat != g->lat)
       return plqcd::set_error(1, "hop: fields and gauge belong to different lattices");
-   if (in->prec != g->prec || out->prec != g->prec)
+   if (in->prec != g->prec || (out->prec != g->prec && !op.resid_b))
       return plqcd::set_error(1, "hop: precision mismatch (gauge %d, in %d, out %d)", g->prec, in->prec, out->prec);
    if (in->nrhs != out->nrhs) return plqcd::set_error(1, "hop: RHS count mismatch");
    if (parity < PLQCD_PARITY_EVEN || parity > PLQCD_PARITY_ALL) return plqcd::set_error(1, "hop: bad parity %d", parity);
@@ -145,6 +145,13 @@ int plqcd::dslash(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip
          if (!f || f->lat != g->lat || f->prec != g->prec || f->nrhs != in->nrhs || f == in || f == out)
             return plqcd::set_error(1, "hop: CG-update operands must be distinct fields like in");
       if (op.fuse_p == op.fuse_x) return plqcd::set_error(1, "hop: CG-update p and x must be distinct");
+   }
+   if (op.resid_b) {
+      const plqcd_hip_field *b = op.resid_b;
+      if (g->prec != PLQCD_PREC_F64 || b->lat != g->lat || b->prec != PLQCD_PREC_F64 || b->nrhs != in->nrhs ||
+          b == in || b == out || op.alpha || !op.dag || !op.kappa || !op.partial || !op.mask)
+         return plqcd::set_error(1, "hop: the residual epilogue needs an F64 gauge, an F64 b distinct from in and "
+                                    "out, M^+, kappa, a norm and a mask");
    }
    switch (g->prec * 100 + g->links) {
       case PLQCD_PREC_F64 * 100 + 18: return plqcd::hop_launch<PLQCD_PREC_F64, 18>(g, in, out, parity, mode, op);

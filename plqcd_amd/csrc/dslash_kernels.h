@@ -27,6 +27,7 @@
 #pragma once
 
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "fields.h"
@@ -415,8 +416,9 @@ __device__ __forceinline__ int logical_block(const HopArgs &a)
 // iterate x and the per-RHS step alpha.
 template <class SIO>
 struct CgFuse {
-   SIO p, x;
+   SIO p, x;  // EPI 3: p holds b
    const double *alpha;
+   const int *mask;  // EPI 3: RHS whose residual is formed
 };
 
 // Epilogue: EPI = 0 stores H in; EPI = 1 stores in(x) - kappa_r * (H in)(x).
@@ -424,10 +426,14 @@ struct CgFuse {
 // CG update at the site instead of storing it: x += alpha_r p, out -= alpha_r
 // Ap (out holds the residual), in the working precision of the kernel (float
 // for F32/F16 storage, double for F64).
+// EPI = 3 (F64 storage) stores the residual b - Ap, with b in f.p, in the
+// precision of `out` for the RHS whose f.mask entry is set (the reliable
+// update of the mixed-precision CG: the true residual goes straight into the
+// low-precision iterate's residual).
 // With NORM the site's |out|^2 joins the deterministic chunk reduction.  `in`
 // and `out` are positioned at the RHS `r`, whose kappa and alpha apply.
-template <int EPI, bool NORM, class SIO>
-__device__ __forceinline__ double epilogue(const SIO &in, const SIO &out, const HopArgs &a, int r, int p, int j,
+template <int EPI, bool NORM, class SIO, class SO>
+__device__ __forceinline__ double epilogue(const SIO &in, const SO &out, const HopArgs &a, int r, int p, int j,
                                            typename SIO::C acc[12], const typename SIO::C *cen, const CgFuse<SIO> &f)
 {
    typedef typename SIO::real real;
@@ -439,7 +445,20 @@ __device__ __forceinline__ double epilogue(const SIO &in, const SIO &out, const 
 #pragma unroll
       for (int m = 0; m < 12; m++) acc[m] = cxmky(cen ? cen[m] : c0[m], kap, acc[m]);
    }
-   if (EPI == 2) {
+   if constexpr (EPI == 3) {
+      if (!f.mask[r]) return 0.0;
+      __builtin_amdgcn_sched_barrier(0);  // b is read after H: no early loads holding registers
+      const SIO b = f.p.at_rhs(r);
+      const uint32_t o = b.off(j);
+      d2 rs[12];
+#pragma unroll
+      for (int m = 0; m < 12; m++) {
+         const C bm = ld<C>(b.cbase(p, m), o);
+         rs[m] = d2{(double)bm.x - (double)acc[m].x, (double)bm.y - (double)acc[m].y};
+      }
+      store_d(out, p, j, rs);
+      return NORM ? norm2(rs) : 0.0;
+   } else if constexpr (EPI == 2) {
       const real al = (real)f.alpha[r];
       {
          C ps[12], xs[12];
@@ -459,19 +478,21 @@ __device__ __forceinline__ double epilogue(const SIO &in, const SIO &out, const 
       }
       out.store(p, j, rs);
       return NORM ? nrm : 0.0;
-   }
-   double nrm = 0.0;
-   if (NORM) {
+   } else {
+      double nrm = 0.0;
+      if (NORM) {
 #pragma unroll
-      for (int m = 0; m < 12; m++) nrm = fma((double)acc[m].y, (double)acc[m].y, fma((double)acc[m].x, (double)acc[m].x, nrm));
+         for (int m = 0; m < 12; m++)
+            nrm = fma((double)acc[m].y, (double)acc[m].y, fma((double)acc[m].x, (double)acc[m].x, nrm));
+      }
+      out.store(p, j, acc);
+      return nrm;
    }
-   out.store(p, j, acc);
-   return nrm;
 }
 
 // One block of the gather kernel: logical block `blk` of the launch.
-template <int MODE, bool DAG, int EPI, bool NORM, int BLOCK, class SIO, class GIO>
-__device__ __forceinline__ void hop_block(const SIO &in, const SIO &out, const GIO &g, const HopArgs &a, int blk,
+template <int MODE, bool DAG, int EPI, bool NORM, int BLOCK, class SIO, class GIO, class SO>
+__device__ __forceinline__ void hop_block(const SIO &in, const SO &out, const GIO &g, const HopArgs &a, int blk,
                                           const CgFuse<SIO> &f)
 {
    typedef typename SIO::C C;
@@ -487,7 +508,8 @@ __device__ __forceinline__ void hop_block(const SIO &in, const SIO &out, const G
    const int j = blk * a.spb + js;
    double nrm = 0.0;
    if (js < a.spb && j < a.Vh) {
-      const SIO si = in.at_rhs(a.r0 + r), so = out.at_rhs(a.r0 + r);
+      const SIO si = in.at_rhs(a.r0 + r);
+      const SO so = out.at_rhs(a.r0 + r);
       const Nbr n = neighbours(a, p, j);
       C acc[12];
 #pragma unroll
@@ -502,8 +524,8 @@ __device__ __forceinline__ void hop_block(const SIO &in, const SIO &out, const G
                                  j - (js & (REDUCE_CHUNK - 1)) < a.Vh, a.partial + (size_t)(a.r0 + r) * 2 * a.nchunk);
 }
 
-template <int MODE, bool DAG, int EPI, bool NORM, int BLOCK, class SIO, class GIO>
-__global__ __launch_bounds__(BLOCK) void hop_kernel(SIO in, SIO out, GIO g, HopArgs a, CgFuse<SIO> f)
+template <int MODE, bool DAG, int EPI, bool NORM, int BLOCK, class SIO, class GIO, class SO>
+__global__ __launch_bounds__(BLOCK) void hop_kernel(SIO in, SO out, GIO g, HopArgs a, CgFuse<SIO> f)
 {
    hop_block<MODE, DAG, EPI, NORM, BLOCK>(in, out, g, a, logical_block(a), f);
 }
@@ -578,7 +600,7 @@ __device__ __forceinline__ void hop_block_rb(const SIO &in, const SIO &out, cons
       for (int k = 0; k < RB; k++)
          chunk_reduce<BLOCK>(nrm[k], js, gi, G, p * a.nchunk + (blk * a.spb + js) / REDUCE_CHUNK,
                              j - (js & (REDUCE_CHUNK - 1)) < a.Vh,
-                             a.partial + (size_t)(a.r0 + gi * RB + k) * 2 * a.nchunk);
+                             a.partial + (size_t)(a.r0 + gi * RB + k) * 2 * a.nchunk, k);
    }
 }
 
@@ -619,8 +641,8 @@ struct BatchList {
 // as the gather kernel does.  The site parity alternates from step to step;
 // the index of the site inside its parity advances by S = L0/2*L1*L2 per
 // t-slice.  Terms are accumulated in the same order as in hop_kernel.
-template <bool DAG, int EPI, bool NORM, int BLOCK, class SIO, class GIO>
-__device__ __forceinline__ void hop_tmarch_block(const SIO &in, const SIO &out, const GIO &g, const HopArgs &a, int lb,
+template <bool DAG, int EPI, bool NORM, int BLOCK, class SIO, class GIO, class SO>
+__device__ __forceinline__ void hop_tmarch_block(const SIO &in, const SO &out, const GIO &g, const HopArgs &a, int lb,
                                                  const CgFuse<SIO> &fuse)
 {
    typedef typename SIO::C C;
@@ -635,7 +657,8 @@ __device__ __forceinline__ void hop_tmarch_block(const SIO &in, const SIO &out, 
    const int js = threadIdx.x / a.R;
    const int c = sb * a.spb + js;
    const bool valid = js < a.spb && c < S;  // a persistent batch block may have spare lanes
-   const SIO si = in.at_rhs(a.r0 + r), so = out.at_rhs(a.r0 + r);
+   const SIO si = in.at_rhs(a.r0 + r);
+   const SO so = out.at_rhs(a.r0 + r);
 
    const int t0 = chunk * a.T;
    const int cc = valid ? c : 0;
@@ -710,12 +733,12 @@ __device__ __forceinline__ void hop_tmarch_block(const SIO &in, const SIO &out, 
          cen = f;
       }
       if (NORM) chunk_reduce<BLOCK>(nrm, js, r, a.R, p * a.nchunk + (sb * a.spb + js + base) / REDUCE_CHUNK, valid,
-                                    a.partial + (size_t)(a.r0 + r) * 2 * a.nchunk);
+                                    a.partial + (size_t)(a.r0 + r) * 2 * a.nchunk, s);
    }
 }
 
-template <bool DAG, int EPI, bool NORM, int BLOCK, class SIO, class GIO>
-__global__ __launch_bounds__(BLOCK) void hop_tmarch_kernel(SIO in, SIO out, GIO g, HopArgs a, CgFuse<SIO> fuse)
+template <bool DAG, int EPI, bool NORM, int BLOCK, class SIO, class GIO, class SO>
+__global__ __launch_bounds__(BLOCK) void hop_tmarch_kernel(SIO in, SO out, GIO g, HopArgs a, CgFuse<SIO> fuse)
 {
    hop_tmarch_block<DAG, EPI, NORM, BLOCK>(in, out, g, a, logical_block(a), fuse);
 }
@@ -918,36 +941,41 @@ __global__ __launch_bounds__(BLOCK) void hop_tile_kernel(SIO in, SIO out, GIO g,
       }
       if (NORM)
          chunk_reduce<BLOCK>(nrm, lc, 0, 1, p * a.nchunk + j / REDUCE_CHUNK, true,
-                             a.partial + (size_t)a.r0 * 2 * a.nchunk);
+                             a.partial + (size_t)a.r0 * 2 * a.nchunk, s);
    }
 }
 
 // kernel variants: gather (1 RHS per lane), t-march, RHS-blocked gather, LDS tile
 enum Variant { V_GATHER = 0, V_MARCH = 1, V_RB2 = 2, V_RB4 = 4, V_TILE = 8 };
 
-template <int MODE, bool DAG, int EPI, bool NORM, int BLOCK, class S, class G>
-void launch_one(int variant, dim3 grid, dim3 block, hipStream_t st, S si, S so, G gg, const HopArgs &a,
+// S: the input field's accessor; SO: the output's (differs only with EPI 3,
+// which runs the gather kernel).
+template <int MODE, bool DAG, int EPI, bool NORM, int BLOCK, class S, class G, class SO>
+void launch_one(int variant, dim3 grid, dim3 block, hipStream_t st, S si, SO so, G gg, const HopArgs &a,
                 const CgFuse<S> &f)
 {
+   constexpr bool same = std::is_same<S, SO>::value && EPI != 3;  // the tile and RHS-blocked kernels
    if (variant == V_MARCH) {
-      if constexpr (MODE == PLQCD_HOP_WILSON)
-         hipLaunchKernelGGL((hop_tmarch_kernel<DAG, EPI, NORM, BLOCK, S, G>), grid, block, 0, st, si, so, gg, a, f);
+      if constexpr (MODE == PLQCD_HOP_WILSON && EPI != 3)
+         hipLaunchKernelGGL((hop_tmarch_kernel<DAG, EPI, NORM, BLOCK, S, G, SO>), grid, block, 0, st, si, so, gg, a, f);
    } else if (variant == V_TILE) {
       // two LDS buffers of BLOCK spinors and BLOCK half-spinors must fit the CU's 160 KiB
-      if constexpr (MODE == PLQCD_HOP_WILSON &&
+      if constexpr (same && MODE == PLQCD_HOP_WILSON &&
                     (2 * sizeof(typename S::Sp) + sizeof(typename S::Hv)) * BLOCK <= 136 * 1024)
          hipLaunchKernelGGL((hop_tile_kernel<DAG, EPI, NORM, BLOCK, S, G>), grid, block, 0, st, si, so, gg, a, f);
    } else if (variant == V_RB2) {
-      hipLaunchKernelGGL((hop_rb_kernel<MODE, DAG, EPI, NORM, BLOCK, 2, S, G>), grid, block, 0, st, si, so, gg, a, f);
+      if constexpr (same)
+         hipLaunchKernelGGL((hop_rb_kernel<MODE, DAG, EPI, NORM, BLOCK, 2, S, G>), grid, block, 0, st, si, so, gg, a, f);
    } else if (variant == V_RB4) {
-      hipLaunchKernelGGL((hop_rb_kernel<MODE, DAG, EPI, NORM, BLOCK, 4, S, G>), grid, block, 0, st, si, so, gg, a, f);
+      if constexpr (same)
+         hipLaunchKernelGGL((hop_rb_kernel<MODE, DAG, EPI, NORM, BLOCK, 4, S, G>), grid, block, 0, st, si, so, gg, a, f);
    } else {
-      hipLaunchKernelGGL((hop_kernel<MODE, DAG, EPI, NORM, BLOCK, S, G>), grid, block, 0, st, si, so, gg, a, f);
+      hipLaunchKernelGGL((hop_kernel<MODE, DAG, EPI, NORM, BLOCK, S, G, SO>), grid, block, 0, st, si, so, gg, a, f);
    }
 }
 
-template <int MODE, bool DAG, int EPI, bool NORM, class S, class G>
-void launch_block(int maxblock, int variant, dim3 grid, dim3 block, hipStream_t st, S si, S so, G gg,
+template <int MODE, bool DAG, int EPI, bool NORM, class S, class G, class SO>
+void launch_block(int maxblock, int variant, dim3 grid, dim3 block, hipStream_t st, S si, SO so, G gg,
                   const HopArgs &a, const CgFuse<S> &f = CgFuse<S>{})
 {
    if (maxblock <= 128) launch_one<MODE, DAG, EPI, NORM, 128>(variant, grid, block, st, si, so, gg, a, f);
@@ -963,15 +991,17 @@ int launch_t(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip_fiel
    HopArgs a = make_hop_args(lat, parity);
    typedef SpinorIO<PREC> S;
    typedef GaugeIO<PREC, LINKS> G;
-   const S si = spinor_io<PREC>(in), so = spinor_io<PREC>(out);
+   const S si = spinor_io<PREC>(in);
    const G gg = gauge_io<PREC, LINKS>(g);
+   const bool resid = op.resid_b != nullptr;  // EPI 3: out may have another precision
    // lanes: the window of right-hand sides [op.r0, op.r0 + R) of the fields
    const int R = op.nr > 0 ? op.nr : in->nrhs;
    a.r0 = op.nr > 0 ? op.r0 : 0;
    HopTuning tu = plqcd::default_tuning(PREC, LINKS, R, op.kappa != nullptr);
    plqcd::lookup_tuning(plqcd::TuneKey{{lat->L[0], lat->L[1], lat->L[2], lat->L[3]}, PREC, LINKS, R}, &tu);
    // RHS per lane (RHS-blocked gather) when it divides R
-   const int rb = (tu.rb == 2 || tu.rb == 4) && R % tu.rb == 0 ? tu.rb : 1;
+   // (the residual epilogue runs the gather kernel only)
+   const int rb = !resid && (tu.rb == 2 || tu.rb == 4) && R % tu.rb == 0 ? tu.rb : 1;
    const int lanes = R / rb;  // lanes per site
    // block shape: `lanes` lanes per site and a whole number of reduction chunks
    int target = tu.block;
@@ -989,30 +1019,38 @@ int launch_t(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip_fiel
    a.partial = op.partial;
    a.nchunk = plqcd::reduce_chunks(lat->Vh);
    const bool dag = op.dag != 0;
-   const int epi = op.alpha ? 2 : op.kappa ? 1 : 0;
+   const int epi = resid ? 3 : op.alpha ? 2 : op.kappa ? 1 : 0;
    const bool norm = op.partial != nullptr;
    if ((dag || epi || norm) && mode != PLQCD_HOP_WILSON)
       return plqcd::set_error(2, "hop: M, M^+ and norms need the wilson mode");
    if (epi == 2 && (!dag || !norm || !op.kappa || !op.fuse_p || !op.fuse_x))
       return plqcd::set_error(2, "hop: the CG-update epilogue needs M^+, a norm, kappa, p and x");
+   if (epi == 3 && (PREC != PLQCD_PREC_F64 || !dag || !norm || !op.kappa || !op.mask))
+      return plqcd::set_error(2, "hop: the residual epilogue needs F64 storage, M^+, a norm, kappa and a mask");
    CgFuse<S> fu{};
    if (epi == 2) {
       fu.p = spinor_io<PREC>(op.fuse_p);
       fu.x = spinor_io<PREC>(op.fuse_x);
       fu.alpha = op.alpha;
+   } else if (epi == 3) {
+      fu.p = spinor_io<PREC>(op.resid_b);
+      fu.mask = op.mask;
    }
    if (norm && a.nparity != 2) return plqcd::set_error(2, "hop: norms are only fused into full-lattice launches");
    const int T = tu.tmarch;
    // the t-march kernel serves full-lattice wilson applications whose time
    // extent is a multiple of T (and, with a norm, whose t-slices hold whole
    // reduction chunks)
-   const bool march = rb == 1 && T > 0 && mode == PLQCD_HOP_WILSON && a.nparity == 2 && lat->L[3] % T == 0 &&
-                      (!norm || a.S % REDUCE_CHUNK == 0);
+   // (not with the residual epilogue: at 32^3 x 64 and 12 RHS that t-march
+   // read 23 GB per launch instead of about 18 GB and took 13 ms, the gather
+   // kernel 4-5 ms)
+   const bool march = !resid && rb == 1 && T > 0 && mode == PLQCD_HOP_WILSON && a.nparity == 2 &&
+                      lat->L[3] % T == 0 && (!norm || a.S % REDUCE_CHUNK == 0);
    // the LDS-tiled t-march: one RHS, a tile of whole x rows dividing the y and
    // z extents, 2 Lxh ty tz lanes per block (at most 256 with F64 storage for
    // the LDS), and, with a norm, reduction chunks that do not leave a tile row
    const int tlanes = 2 * lat->Lxh * tu.ty * tu.tz;
-   const bool tiled = march && R == 1 && tu.ty > 0 && tu.tz > 0 && lat->L[1] % tu.ty == 0 &&
+   const bool tiled = march && !resid && R == 1 && tu.ty > 0 && tu.tz > 0 && lat->L[1] % tu.ty == 0 &&
                       lat->L[2] % tu.tz == 0 &&
                       (tlanes == 128 || tlanes == 256 || (tlanes == 512 && PREC != PLQCD_PREC_F64)) &&
                       (!norm || (lat->Lxh * tu.ty) % REDUCE_CHUNK == 0);
@@ -1034,6 +1072,22 @@ int launch_t(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip_fiel
    if ((a.order == 1 && nb % 8) || (a.order == 2 && nb % (8 * a.group))) a.order = 0;
    const dim3 grid(nb), block(nthreads);
    hipStream_t st = lat->stream;
+   if (epi == 3) {
+      if constexpr (PREC == PLQCD_PREC_F64) {
+         if (out->prec == PLQCD_PREC_F64)
+            launch_block<PLQCD_HOP_WILSON, true, 3, true>(mblock, variant, grid, block, st, si,
+                                                           spinor_io<PLQCD_PREC_F64>(out), gg, a, fu);
+         else if (out->prec == PLQCD_PREC_F32)
+            launch_block<PLQCD_HOP_WILSON, true, 3, true>(mblock, variant, grid, block, st, si,
+                                                           spinor_io<PLQCD_PREC_F32>(out), gg, a, fu);
+         else
+            launch_block<PLQCD_HOP_WILSON, true, 3, true>(mblock, variant, grid, block, st, si,
+                                                           spinor_io<PLQCD_PREC_F16>(out), gg, a, fu);
+      }
+      PLQCD_HIP_TRY(hipGetLastError());
+      return 0;
+   }
+   const S so = spinor_io<PREC>(out);
    if (mode == PLQCD_HOP_COMPAT) {
       launch_block<PLQCD_HOP_COMPAT, false, 0, false>(mblock, variant, grid, block, st, si, so, gg, a);
    } else if (!dag) {

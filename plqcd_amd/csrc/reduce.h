@@ -24,20 +24,25 @@ int reduce_partials(double *partial, size_t stride, int first, int n, int R, dou
 
 // Stage 1 inside a block whose lanes are (js, r), tid = js * R + r, with
 // REDUCE_CHUNK | (sites per block).  Every thread of the block must call it.
-// The lane holding the first site of a chunk writes the chunk sum to
-// out[chunk] when `write` is set.
+// The lane holding the first site of a chunk sums the chunk's 8 site values
+// in the fixed tree ((v0 + v4) + (v2 + v6)) + ((v1 + v5) + (v3 + v7)) and
+// writes the sum to out[chunk] when `write` is set.  One barrier per call:
+// consecutive calls of a block alternate between two LDS buffers (`phase`
+// counts the calls), so a call's writes cannot reach a buffer whose reads
+// are still pending.
 template <int BLOCK>
-__device__ __forceinline__ void chunk_reduce(double v, int js, int r, int R, int chunk, bool write, double *out)
+__device__ __forceinline__ void chunk_reduce(double v, int js, int r, int R, int chunk, bool write, double *out,
+                                             int phase = 0)
 {
-   __shared__ double red[BLOCK];
+   __shared__ double red[2][BLOCK];
+   double *b = red[phase & 1];
    const int tid = threadIdx.x;
-   red[tid] = v;
+   b[tid] = v;
    __syncthreads();
-#pragma unroll
-   for (int s = REDUCE_CHUNK / 2; s > 0; s >>= 1) {
-      if ((js & (REDUCE_CHUNK - 1)) < s) red[tid] += red[tid + s * R];
-      __syncthreads();
+   if ((js & (REDUCE_CHUNK - 1)) == 0 && write) {
+      const double *c = b + tid;
+      const double lo = (c[0] + c[4 * R]) + (c[2 * R] + c[6 * R]);
+      const double hi = (c[R] + c[5 * R]) + (c[3 * R] + c[7 * R]);
+      out[chunk] = lo + hi;
    }
-   if ((js & (REDUCE_CHUNK - 1)) == 0 && write) out[chunk] = red[tid];
-   __syncthreads();  // red is reused by the next call
 }

@@ -73,7 +73,7 @@ size_t CgEngine::bytes_needed(const plqcd_hip_lattice *lat, int sloppy, int S)
 {
    const size_t f64 = 2 * spinor_parity_bytes(PLQCD_PREC_F64, lat->Vh) * S;
    const size_t fs = 2 * spinor_parity_bytes(sloppy, lat->Vh) * S;
-   const size_t nf64 = sloppy == PLQCD_PREC_F64 ? 3 : 4;  // b, x, t2 (+ t1)
+   const size_t nf64 = sloppy == PLQCD_PREC_F64 ? 2 : 3;  // b, x (+ t1)
    return nf64 * f64 + 4 * fs + (size_t)S * 2 * reduce_chunks(lat->Vh) * sizeof(double) + (1 << 20);
 }
 
@@ -126,7 +126,6 @@ int CgEngine::alloc(int sloppy)
    } else {
       if (int rc = plqcd_hip_field_create(lat_, PLQCD_PREC_F64, S_, &t1_)) return rc;
    }
-   if (int rc = plqcd_hip_field_create(lat_, PLQCD_PREC_F64, S_, &t2_)) return rc;
    PLQCD_HIP_TRY(hipMalloc(&partial_, partial_doubles(b_) * sizeof(double)));
    PLQCD_HIP_TRY(hipMalloc(&scal_, 7 * S_ * sizeof(double)));
    PLQCD_HIP_TRY(hipMalloc(&imask_, 2 * S_ * sizeof(int)));
@@ -155,7 +154,6 @@ CgEngine::~CgEngine()
    const bool alias = (t1_ == qs_);
    for (plqcd_hip_field *f : {rs_, ps_, qs_, xs_}) plqcd_hip_field_destroy(f);
    if (!alias) plqcd_hip_field_destroy(t1_);
-   plqcd_hip_field_destroy(t2_);
    if (own_bx_) {
       plqcd_hip_field_destroy(b_);
       plqcd_hip_field_destroy(x_);
@@ -347,11 +345,14 @@ int CgEngine::reliable_update()
    m.nr = w.nr;
    m.kappa = kappa_d();
    if (int rc = dslash(g_, x_, t1_, PLQCD_PARITY_ALL, PLQCD_HOP_WILSON, m)) return rc;
+   // r = b - M^+ t1 in FP64, stored straight into the iterate's residual in
+   // its own precision, with |r|^2, for the RHS being updated
    m.dag = 1;
-   if (int rc = dslash(g_, t1_, t2_, PLQCD_PARITY_ALL, PLQCD_HOP_WILSON, m)) return rc;
-   if (int rc = blas_residual(b_, t2_, t2_, partial_, update_d(), w)) return rc;
+   m.partial = partial_;
+   m.resid_b = b_;
+   m.mask = update_d();
+   if (int rc = dslash(g_, t1_, rs_, PLQCD_PARITY_ALL, PLQCD_HOP_WILSON, m)) return rc;
    if (int rc = reduce_partials(partial_, 2 * nc, 0, 2 * nc, w.nr, tmp_d(), st, w.r0)) return rc;
-   if (int rc = blas_copy(rs_, t2_, nullptr, update_d(), w)) return rc;
    PLQCD_HIP_TRY(hipMemcpyAsync(host_ + S_, tmp_d(), S_ * sizeof(double), hipMemcpyDeviceToHost, st));
    return 0;
 }
