@@ -303,6 +303,14 @@ def calib_suite():
         capi.check(lib.plqcd_hip_stream_bandwidth(kind, 1 << 30, 10, C.byref(gbs)), "stream_bandwidth")
         emit({"suite": "calib", "case": f"stream_{name}_1GiB", "bytes_per_buffer": 1 << 30, "timed_launches": 10,
               "GBps": round(gbs.value, 1)})
+    # the same probes over working sets that fit the 256 MB Infinity Cache
+    # (read 128 MiB, copy 2 x 128 MiB) and that do not (4 GiB)
+    for kind, name, mib, reps in ((1, "read", 128, 100), (0, "copy", 128, 100), (1, "read", 4096, 10),
+                                  (0, "copy", 4096, 10)):
+        gbs = C.c_double()
+        capi.check(lib.plqcd_hip_stream_bandwidth(kind, mib << 20, reps, C.byref(gbs)), "stream_bandwidth")
+        emit({"suite": "calib", "case": f"stream_{name}_{mib}MiB", "bytes_per_buffer": mib << 20,
+              "timed_launches": reps, "GBps": round(gbs.value, 1)})
     L = (32, 32, 32, 64)
     V = int(np.prod(L))
     u = hp.random_gauge_su3(V, 1)
