@@ -2,6 +2,8 @@
 // the kernels that repack the reference's host layout (array of structures in
 // even-odd site order, include/su3.h:52-75) into the structure-of-arrays HBM
 // layouts described in device_common.h, converting precision on the way.
+#include <sys/mman.h>
+
 #include <algorithm>
 #include <cstring>
 #include <new>
@@ -479,9 +481,28 @@ int plqcd::Stager::upload_raw(void *dev, const void *host, size_t bytes)
    return 0;
 }
 
+namespace {
+#ifndef MADV_POPULATE_WRITE
+#define MADV_POPULATE_WRITE 23
+#endif
+// Maps the whole pages of a host destination in one call before they are
+// written: the first write to a fresh page otherwise takes a fault and a
+// zeroing per page (a fresh 2 GB array filled at ~1.5-3.5 GB/s instead of
+// ~9 GB/s).  The data is not touched; kernels without the call return an
+// error, which changes nothing.
+void prefault(void *p, size_t n)
+{
+   const uintptr_t pg = 4096;
+   const uintptr_t a = (reinterpret_cast<uintptr_t>(p) + pg - 1) & ~(pg - 1);
+   const uintptr_t b = (reinterpret_cast<uintptr_t>(p) + n) & ~(pg - 1);
+   if (b > a) (void)madvise(reinterpret_cast<void *>(a), b - a, MADV_POPULATE_WRITE);
+}
+}  // namespace
+
 int plqcd::Stager::download_raw(void *host, const void *dev, size_t bytes, hipEvent_t after)
 {
    if (!dev || !host) return set_error(1, "stager: null pointer");
+   prefault(host, bytes);  // while the solution may still be unpacking
    // a host-side wait: a stream wait on `after` would tie this stream to the
    // other stream, which fails while that stream is being captured
    if (after) PLQCD_HIP_TRY(hipEventSynchronize(after));

@@ -85,6 +85,7 @@ struct Entry {
    std::vector<std::pair<Request *, int>> owner;
    enum SlotState { FREE, UPLOADING, UPLOADED, ACTIVE, DOWNLOADING };
    std::vector<int> sstate;
+   std::vector<int> parts, part_rc;     // per slot: transfer parts still out, their combined status
    char *io = nullptr;                  // per slot: V spinors in the host layout (device)
    std::vector<hipEvent_t> io_ev;       // per slot: the solution is in io
    char *io_slot(int c) const { return io + (size_t)c * lat->V * sizeof(spinor); }
@@ -109,6 +110,11 @@ struct Entry {
 
 // A host transfer job: the source of `rhs` of request r into slot `slot`
 // of the entry's engine (up), or that slot's solution into the request (down).
+// One transfer job: bytes [off, off + len) of one RHS between the request's
+// host array and the slot's device buffer (both in the host layout).  A large
+// RHS is split into parts that several transfer threads move at once: a
+// solution lands in host pages the caller has just allocated, and the kernel
+// zeroes each page on its first write at a few GB/s per core.
 struct Xfer {
    bool down = false;
    Entry *e = nullptr;
@@ -116,7 +122,9 @@ struct Xfer {
    Request *r = nullptr;
    int rhs = 0;
    int rc = 0;
+   size_t off = 0, len = 0;
 };
+constexpr size_t kXferPart = size_t(256) << 20;  // split above this many bytes per RHS
 
 struct plqcd_hip_server {
    plqcd_serve_config cfg{};
@@ -298,6 +306,8 @@ Entry *entry_for(plqcd_hip_server *s, Request *r)
       e->engine_bytes = eb;
       e->owner.assign(S, {nullptr, 0});
       e->sstate.assign(S, Entry::FREE);
+      e->parts.assign(S, 0);
+      e->part_rc.assign(S, 0);
       s->used += eb;
    }
    return e;
@@ -320,11 +330,23 @@ void complete(plqcd_hip_server *s, Request *r)
    s->cv.notify_all();
 }
 
-void push_xfer(plqcd_hip_server *s, const Xfer &j)
+// Queues the transfer of one RHS of slot `slot` as up to one part per
+// transfer thread (parts of at least kXferPart bytes, whole spinors).
+void push_xfer(plqcd_hip_server *s, bool down, Entry *e, int slot, Request *r, int rhs)
 {
+   const size_t bytes = (size_t)e->lat->V * sizeof(spinor);
+   const size_t nt = std::max<size_t>(1, s->xfer.size());
+   const size_t k = std::min(nt, std::max<size_t>(1, bytes / kXferPart));
+   const size_t per = ((bytes / k + sizeof(spinor) - 1) / sizeof(spinor)) * sizeof(spinor);
+   e->parts[slot] = 0;
+   e->part_rc[slot] = 0;
    std::lock_guard<std::mutex> lk(s->xmu);
-   s->xjobs.push_back(j);
-   s->xcv.notify_one();
+   for (size_t o = 0; o < bytes; o += per) {
+      Xfer j{down, e, slot, r, rhs, 0, o, std::min(per, bytes - o)};
+      e->parts[slot]++;
+      s->xjobs.push_back(j);
+   }
+   s->xcv.notify_all();
 }
 
 // Starts the uploads of the pending right-hand sides of waiting requests into
@@ -342,7 +364,7 @@ void admit_pending(plqcd_hip_server *s)
          e->owner[c] = {r, r->next_rhs};
          if (r->t_start < 0) r->t_start = s->now();
          r->inflight++;
-         push_xfer(s, Xfer{false, e, c, r, r->next_rhs, 0});
+         push_xfer(s, false, e, c, r, r->next_rhs);
          r->next_rhs++;
       }
    }
@@ -358,12 +380,15 @@ void collect_xfers(plqcd_hip_server *s)
    }
    for (const Xfer &j : done) {
       Entry *e = j.e;
+      if (j.rc) e->part_rc[j.slot] = j.rc;
+      if (--e->parts[j.slot] > 0) continue;  // the slot's other parts are still moving
+      const int rc = e->part_rc[j.slot];
       if (j.r) j.r->inflight--;
-      if (j.rc) {
+      if (rc) {
          if (j.r) j.r->status = -1;  // reported by the transfer thread
       }
       if (!j.down) {
-         if (j.rc || !j.r || j.r->status != 0) {
+         if (rc || !j.r || j.r->status != 0) {
             e->sstate[j.slot] = Entry::FREE;
             e->owner[j.slot] = {nullptr, 0};
          } else {
@@ -392,14 +417,15 @@ void xfer_main(plqcd_hip_server *s)
       }
       const size_t V = (size_t)j.e->lat->V;
       const auto t0 = clock_t_::now();
+      char *dev = j.e->io_slot(j.slot) + j.off;
       if (rc0) j.rc = rc0;  // reported below
       else if (j.down)
-         j.rc = j.r ? g->download_raw(j.r->req.x + j.rhs * V, j.e->io_slot(j.slot), V * sizeof(spinor),
+         j.rc = j.r ? g->download_raw(reinterpret_cast<char *>(j.r->req.x + j.rhs * V) + j.off, dev, j.len,
                                       j.e->io_ev[j.slot])
                     : 0;
-      else j.rc = g->upload_raw(j.e->io_slot(j.slot), j.r->req.b + j.rhs * V, V * sizeof(spinor));
+      else j.rc = g->upload_raw(dev, reinterpret_cast<const char *>(j.r->req.b + j.rhs * V) + j.off, j.len);
       if (!j.rc && (j.r || !j.down)) {
-         atomic_add(s->xfer_bytes, (double)V * sizeof(spinor));
+         atomic_add(s->xfer_bytes, (double)j.len);
          atomic_add(s->xfer_seconds, std::chrono::duration<double>(clock_t_::now() - t0).count());
       }
       if (j.rc) note_failure(s, j.down ? "solution download" : "source upload");
@@ -440,7 +466,7 @@ void retire_finished(plqcd_hip_server *s, Entry *e, const std::vector<int> &fin)
       }
       e->sstate[slot] = Entry::DOWNLOADING;
       r->inflight++;
-      push_xfer(s, Xfer{true, e, slot, r, e->owner[slot].second, 0});
+      push_xfer(s, true, e, slot, r, e->owner[slot].second);
    }
 }
 

@@ -58,3 +58,23 @@ def test_admission_by_budget_and_reuse(hp):
         rid = s.submit((16, 16, 16, 16), 1, source(hp, (16, 16, 16, 16), 1, 0), 0.12, 1e-9)
         res = s.run_all()
     assert res[rid]["status"] == -2
+
+
+def test_large_rhs_moves_in_parts(hp):
+    """A right-hand side of more than 256 MiB (32^3x96: 604 MB) is moved by
+    several transfer threads at once, each a part of it; the solution is the
+    same as with one thread moving it whole."""
+    from plqcd_amd.serve import Server
+    L = (32, 32, 32, 96)
+    b = source(hp, L, 2, 7)
+    out = []
+    for threads in (1, 8):
+        with Server(slots=2, transfer_threads=threads) as s:
+            rid = s.submit(L, 3, b, 0.12, 1e-6)
+            res = s.run_all()[rid]
+            info = s.info()
+        assert info["failed"] == 0 and res["status"] == 0 and res["converged"]
+        assert info["transfer_bytes"] == 2 * b.nbytes
+        out.append(res["x"])
+    assert np.array_equal(out[0], out[1])
+    assert np.isfinite(out[1]).all() and np.abs(out[1]).max() > 0
