@@ -325,6 +325,18 @@ def calib_suite():
         emit({"suite": "calib", "case": f"hop_{PREC_NAMES[prec]}_l{links}_32x32x32x64", "applications": 2,
               "compulsory_bytes": BYTES_PER_SITE[(prec, links)] * V})
         fi.close(); fo.close(); g.close()
+    # FP64 M = 1 - kappa H on 12 RHS (the reliable update's operator), twice
+    from plqcd_amd import solver as sv
+    g = hp.Gauge(lat, 0, 18).upload(u)
+    fi, fo = hp.Field(lat, 0, 12), hp.Field(lat, 0, 12)
+    for r in range(12):
+        fi.upload(hp.random_spinor(V, 10 + r), r)
+    for _ in range(2):
+        sv.apply_m(g, fi, fo, 0.124)
+    lat.sync()
+    emit({"suite": "calib", "case": "apply_m_f64_l18_32x32x32x64_R12", "applications": 2,
+          "compulsory_bytes": (12 * (192 + 192) + 576) * V})
+    fi.close(); fo.close(); g.close()
     lat.close()
 
 
@@ -419,8 +431,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solver", action="store_true")
     ap.add_argument("--no-formats", action="store_true", help="skip the FP32/FP16 storage-format timings")
-    ap.add_argument("--suite", choices=["hop", "solver", "serve", "calib"], default="hop",
-                    help="hop: the headline line (default); solver/serve: the extended studies, one JSON line each")
+    ap.add_argument("--suite", choices=["hop", "solver", "serve", "calib", "cg"], default="hop",
+                    help="hop: the headline line (default); solver/serve: the extended studies, one JSON line "
+                         "each; cg: the bench line's solver comparison alone")
     ap.add_argument("--duration", type=float, default=60.0, help="serve suite: seconds of Poisson arrivals")
     ap.add_argument("--rate", type=float, default=2.0, help="serve suite: requests per second")
     args = ap.parse_args()
@@ -437,6 +450,11 @@ def main():
         return
     if args.suite == "calib":
         calib_suite()
+        return
+    if args.suite == "cg":
+        rec = solve_compare(tuple(int(v) for v in args.lattice.split(",")))
+        rec.update({"suite": "cg"})
+        emit(rec)
         return
 
     L = tuple(int(x) for x in args.lattice.split(","))
