@@ -340,16 +340,23 @@ def calib_suite():
     lat.close()
 
 
-def serve_suite(duration: float, rate: float, seed: int = 7):
+def serve_suite(duration: float, rate: float, seed: int = 7, pinned: bool = False):
     """Poisson request stream for `duration` seconds; volumes from
     {16^4, 24^3x48, 32^3x64, 48^3x96}, 1..12 RHS, tol in {1e-8,1e-10,1e-12},
     kappa in [0.120, 0.1245], two gauge fields per volume.  Reports p50/p99
-    latency and solves/s beside one-at-a-time plain FP64 solves."""
+    latency and solves/s beside one-at-a-time plain FP64 solves.  With `pinned`
+    the sources live in page-locked host memory (moved by DMA alone)."""
     from plqcd_amd import hopping as hp, solver as sv
     from plqcd_amd.serve import Server
     rng = np.random.default_rng(seed)
     vols = [(16, 16, 16, 16), (24, 24, 24, 48), (32, 32, 32, 64), (48, 48, 48, 96)]
     pool = {L: np.stack([hp.random_spinor(int(np.prod(L)), 1000 + k) for k in range(12)]) for L in vols}
+    if pinned:
+        from plqcd_amd.serve import pinned_empty
+        for L in vols:
+            pp = pinned_empty(pool[L].shape)
+            pp[...] = pool[L]
+            pool[L] = pp
     arrivals = np.cumsum(rng.exponential(1.0 / rate, size=int(rate * duration * 2) + 10))
     arrivals = arrivals[arrivals < duration]
     reqs = []
@@ -404,6 +411,7 @@ def serve_suite(duration: float, rate: float, seed: int = 7):
     base_seconds = sum(per_rhs[submitted[i]["L"]] * submitted[i]["nrhs"] for i in done)
     base_io_seconds = sum(per_rhs_io[submitted[i]["L"]] * submitted[i]["nrhs"] for i in done)
     emit({"suite": "serve", "case": "poisson_mixed_volumes", "duration_s": duration, "rate_per_s": rate,
+          "pinned_sources": pinned,
           "requests": len(done), "rhs_solved": nrhs, "all_converged": ok, "wall_s": round(wall, 2),
           "p50_latency_s": round(float(np.percentile(lat_s, 50)), 4),
           "p99_latency_s": round(float(np.percentile(lat_s, 99)), 4),
@@ -436,6 +444,7 @@ def main():
                          "each; cg: the bench line's solver comparison alone")
     ap.add_argument("--duration", type=float, default=60.0, help="serve suite: seconds of Poisson arrivals")
     ap.add_argument("--rate", type=float, default=2.0, help="serve suite: requests per second")
+    ap.add_argument("--pinned", action="store_true", help="serve suite: sources in page-locked host memory")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): the hopping matrix here is a "
@@ -446,7 +455,7 @@ def main():
         solver_suite()
         return
     if args.suite == "serve":
-        serve_suite(args.duration, args.rate)
+        serve_suite(args.duration, args.rate, pinned=args.pinned)
         return
     if args.suite == "calib":
         calib_suite()

@@ -352,6 +352,14 @@ int plqcd_hip_hop_batch(int n, const plqcd_hip_gauge *const *g, const plqcd_hip_
  * and writes `bytes`); 1: read only; 2: write only. */
 int plqcd_hip_stream_bandwidth(int kind, size_t bytes, int reps, double *gbs);
 
+/* ------------------------------------------------ host memory */
+/* Page-locked host memory (hipHostMalloc).  The server moves requests'
+ * sources and solutions that lie in such a buffer by DMA alone, without the
+ * copy through its own pinned staging buffers (no counterpart in the
+ * reference, which has no host/device transfers). */
+int plqcd_hip_host_alloc(size_t bytes, void **out);
+void plqcd_hip_host_free(void *p);
+
 /* ------------------------------------------------ synthetic inputs */
 /* Seeded, counter-based and thread-order independent: the value at each
  * (site, component) depends only on (seed, even-odd site index, component).

@@ -105,6 +105,8 @@ _PROTOS = {
     "plqcd_hip_hop_get_tuning": (C.c_int, [C.c_int, C.POINTER(C.c_int)]),
     "plqcd_hip_hop_tuning_reset": (C.c_int, [C.c_int]),
     "plqcd_hip_stream_bandwidth": (C.c_int, [C.c_int, C.c_size_t, C.c_int, C.POINTER(C.c_double)]),
+    "plqcd_hip_host_alloc": (C.c_int, [C.c_size_t, C.POINTER(vp)]),
+    "plqcd_hip_host_free": (None, [vp]),
     "plqcd_random_gauge_su3": (None, [vp, C.c_int, C.c_uint64]),
     "plqcd_random_spinor_gauss": (None, [vp, C.c_int, C.c_uint64]),
     "plqcd_unit_gauge": (None, [vp, C.c_int]),

@@ -6,6 +6,8 @@
 
 #include <algorithm>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <new>
 #include <type_traits>
 
@@ -464,9 +466,59 @@ int plqcd::Stager::download(const plqcd_hip_field *f, int rhs, spinor *host)
    return 0;
 }
 
+// ------------------------------------------------------- pinned host memory
+//
+// Host buffers from plqcd_hip_host_alloc are page-locked: the stager moves
+// them by DMA alone, without the copy through its own pinned buffers.  The
+// allocations are recorded so that a transfer can tell whether its whole
+// host range lies in one of them.
+namespace {
+std::mutex g_pin_mu;
+std::map<uintptr_t, size_t> g_pinned;  // base -> bytes
+}  // namespace
+
+bool plqcd::host_pinned(const void *p, size_t n)
+{
+   const uintptr_t a = reinterpret_cast<uintptr_t>(p);
+   std::lock_guard<std::mutex> lk(g_pin_mu);
+   auto it = g_pinned.upper_bound(a);
+   if (it == g_pinned.begin()) return false;
+   --it;
+   return a >= it->first && a + n <= it->first + it->second;
+}
+
+extern "C" int plqcd_hip_host_alloc(size_t bytes, void **out)
+{
+   plqcd::clear_error();
+   if (!out || bytes == 0) return plqcd::set_error(1, "plqcd_hip_host_alloc: bad arguments");
+   *out = nullptr;
+   void *p = nullptr;
+   const hipError_t e = hipHostMalloc(&p, bytes, hipHostMallocDefault);
+   if (e != hipSuccess) return plqcd::set_error(4, "plqcd_hip_host_alloc(%zu): %s", bytes, hipGetErrorString(e));
+   std::lock_guard<std::mutex> lk(g_pin_mu);
+   g_pinned[reinterpret_cast<uintptr_t>(p)] = bytes;
+   *out = p;
+   return 0;
+}
+
+extern "C" void plqcd_hip_host_free(void *p)
+{
+   if (!p) return;
+   {
+      std::lock_guard<std::mutex> lk(g_pin_mu);
+      g_pinned.erase(reinterpret_cast<uintptr_t>(p));
+   }
+   (void)hipHostFree(p);
+}
+
 int plqcd::Stager::upload_raw(void *dev, const void *host, size_t bytes)
 {
    if (!dev || !host) return set_error(1, "stager: null pointer");
+   if (host_pinned(host, bytes)) {  // DMA straight from the caller's buffer
+      PLQCD_HIP_TRY(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, st_));
+      PLQCD_HIP_TRY(hipStreamSynchronize(st_));
+      return 0;
+   }
    bool used[2] = {false, false};
    for (size_t o = 0, c = 0; o < bytes; o += kStagerChunk, c++) {
       const int b = (int)(c & 1);
@@ -502,6 +554,12 @@ void prefault(void *p, size_t n)
 int plqcd::Stager::download_raw(void *host, const void *dev, size_t bytes, hipEvent_t after)
 {
    if (!dev || !host) return set_error(1, "stager: null pointer");
+   if (host_pinned(host, bytes)) {  // DMA straight into the caller's buffer
+      if (after) PLQCD_HIP_TRY(hipEventSynchronize(after));
+      PLQCD_HIP_TRY(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, st_));
+      PLQCD_HIP_TRY(hipStreamSynchronize(st_));
+      return 0;
+   }
    prefault(host, bytes);  // while the solution may still be unpacking
    // a host-side wait: a stream wait on `after` would tie this stream to the
    // other stream, which fails while that stream is being captured

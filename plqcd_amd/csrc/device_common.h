@@ -187,12 +187,16 @@ __device__ __forceinline__ uint32_t f2_to_h2(float a, float b)
 
 namespace plqcd {
 
+// Whether [p, p + n) lies inside one buffer from plqcd_hip_host_alloc.
+bool host_pinned(const void *p, size_t n);
+
 // Host <-> device transfers of one RHS of a field on a stream of their own,
 // through two pinned host buffers and two device buffers: the CPU copy of one
 // chunk overlaps the DMA and (un)packing of the previous one, and the field's
 // lattice stream keeps running.  The caller keeps kernels on other streams
 // away from the RHS being transferred.  Each call returns when its transfer
-// is complete; one Stager serves one thread at a time.
+// is complete; one Stager serves one thread at a time.  The raw copies move a
+// host range from plqcd_hip_host_alloc by DMA alone.
 class Stager {
   public:
    static int create(int device, Stager **out);

@@ -6,12 +6,26 @@ the host buffers of in-flight requests alive and turns results into dicts.
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 
 import numpy as np
 
 from . import capi
 from .capi import check, lib
 from .hopping import PREC_F16, PREC_F32, PREC_F64  # noqa: F401
+
+
+def pinned_empty(shape, dtype=np.complex128) -> np.ndarray:
+    """An uninitialised array in page-locked host memory
+    (plqcd_hip_host_alloc), freed when the last view of it is gone.  The
+    server moves sources and solutions held in such arrays by DMA alone."""
+    n = int(np.prod(shape))
+    nbytes = max(1, n * np.dtype(dtype).itemsize)
+    p = C.c_void_p()
+    check(lib().plqcd_hip_host_alloc(nbytes, C.byref(p)), "host_alloc")
+    buf = (C.c_char * nbytes).from_address(p.value)
+    weakref.finalize(buf, lib().plqcd_hip_host_free, C.c_void_p(p.value))
+    return np.frombuffer(buf, dtype=dtype, count=n).reshape(shape)
 
 
 class Server:
@@ -25,13 +39,20 @@ class Server:
         self._live = {}  # id -> (b, x, request struct)
 
     def submit(self, L, gauge_seed: int, b: np.ndarray, kappa: float, tol: float, max_iter: int = 20000,
-               gauge: np.ndarray | None = None) -> int:
-        """b: complex128 array (nrhs, V, 4, 3).  Returns the request id."""
+               gauge: np.ndarray | None = None, out: np.ndarray | None = None) -> int:
+        """b: complex128 array (nrhs, V, 4, 3).  out: optional array of the same
+        shape for the solutions (for instance from pinned_empty), else a new
+        one.  Returns the request id."""
         V = int(np.prod(L))
         b = np.ascontiguousarray(b, dtype=np.complex128).reshape(-1, V, 4, 3)
-        # the server writes every element; empty_like commits no host pages
-        # until the solution arrives (zeros_like would touch them all now)
-        x = np.empty_like(b)
+        if out is None:
+            # the server writes every element; empty_like commits no host pages
+            # until the solution arrives (zeros_like would touch them all now)
+            x = np.empty_like(b)
+        else:
+            if out.shape != b.shape or out.dtype != np.complex128 or not out.flags.c_contiguous:
+                raise ValueError(f"out must be a C-contiguous complex128 array of shape {b.shape}")
+            x = out
         req = capi.ServeRequest()
         req.L = (C.c_int * 4)(*L)
         req.gauge_seed = gauge_seed

@@ -78,3 +78,24 @@ def test_large_rhs_moves_in_parts(hp):
         out.append(res["x"])
     assert np.array_equal(out[0], out[1])
     assert np.isfinite(out[1]).all() and np.abs(out[1]).max() > 0
+
+
+def test_pinned_buffers_move_by_dma(hp):
+    """Sources and solutions in page-locked host memory (pinned_empty) take
+    the DMA-only path; the solution equals the one through pageable memory."""
+    from plqcd_amd.serve import Server, pinned_empty
+    L = (8, 8, 8, 16)
+    b = source(hp, L, 3, 21)
+    bp = pinned_empty(b.shape)
+    bp[...] = b
+    xp = pinned_empty(b.shape)
+    with Server(slots=4) as s:
+        r1 = s.submit(L, 5, b, 0.123, 1e-9)
+        r2 = s.submit(L, 5, bp, 0.123, 1e-9, out=xp)
+        res = s.run_all()
+    assert res[r1]["status"] == 0 and res[r2]["status"] == 0
+    assert res[r2]["x"] is xp
+    assert np.array_equal(res[r1]["x"], xp)
+    with pytest.raises(ValueError):
+        with Server(slots=1) as s:
+            s.submit(L, 5, b, 0.123, 1e-9, out=np.empty((1,) + b.shape[1:], np.complex128))
