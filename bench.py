@@ -50,7 +50,7 @@ def log(*a):
 
 
 def read_traffic(kernel_tag: str):
-    """HBM bytes per launch measured with rocprofv3 --pmc, from profiles/."""
+    """L2-to-memory bytes per launch measured with rocprofv3 --pmc, from profiles/."""
     f = ROOT / "profiles" / f"{kernel_tag}_pmc.txt"
     if not f.exists():
         return None
