@@ -9,9 +9,11 @@
 
 // Built-in launch shapes, from sweeps at 32^3 x 64.  One RHS, per storage
 // format: the LDS-tiled t-march for F64 (tiles of 32 x 4 x 1 sites, 32
-// t-slices: 434 us against 514 us for the untiled t-march) and F32 (32 x 4 x 4
-// with 12-real links: 174 against 217 us; 32 x 4 x 2 with 18-real links, whose
-// 512-lane tile spills registers), the untiled t-march over 4 t-slices for F16
+// t-slices: 434 us against 514 us for the untiled t-march; 32 x 1 x 4 with
+// 12-real links: 351 against 356 us for 32 x 4 x 1) and F32 (32 x 8 x 1 sites
+// and 32 t-slices with 18-real links: 247 against 264 us for 32 x 4 x 2 and 8
+// t-slices; 32 x 4 x 4 and 8 t-slices with 12-real links: 171 us), the
+// untiled t-march over 4 t-slices for F16
 // (102 us with 12-real links, the gather kernel 116 us).  Shapes that do not
 // fit a lattice fall back to the untiled t-march or the gather kernel.  Several
 // RHS per launch, per precision: XCDs sweeping the lattice together, with
@@ -22,8 +24,8 @@ struct FormatShape {
    HopTuning t;
 };
 const FormatShape kOneRhs[6] = {{PLQCD_PREC_F64, 18, {1, 1, 256, 32, 1, 4, 1}},
-                                {PLQCD_PREC_F64, 12, {1, 1, 256, 32, 1, 4, 1}},
-                                {PLQCD_PREC_F32, 18, {2, 8, 256, 8, 1, 4, 2}},
+                                {PLQCD_PREC_F64, 12, {1, 1, 256, 32, 1, 1, 4}},
+                                {PLQCD_PREC_F32, 18, {1, 1, 256, 32, 1, 8, 1}},
                                 {PLQCD_PREC_F32, 12, {2, 8, 256, 8, 1, 4, 4}},
                                 {PLQCD_PREC_F16, 12, {1, 1, 256, 4, 1, 0, 0}},
                                 {PLQCD_PREC_F16, 18, {2, 8, 512, 4, 1, 0, 0}}};
