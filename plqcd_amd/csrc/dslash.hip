@@ -13,8 +13,9 @@
 // 12-real links: 351 against 356 us for 32 x 4 x 1) and F32 (32 x 8 x 1 sites
 // and 32 t-slices with 18-real links: 247 against 264 us for 32 x 4 x 2 and 8
 // t-slices; 32 x 4 x 4 and 8 t-slices with 12-real links: 171 us), the
-// untiled t-march over 4 t-slices for F16
-// (102 us with 12-real links, the gather kernel 116 us).  Shapes that do not
+// untiled t-march over 8 t-slices in 512-lane blocks for F16 (102 us with
+// 12-real links, the gather kernel 116 us; 118 against 124 us with 18-real
+// links for 4 t-slices).  Shapes that do not
 // fit a lattice fall back to the untiled t-march or the gather kernel.  Several
 // RHS per launch, per precision: XCDs sweeping the lattice together, with
 // short t-marches (F16 at 12 RHS: 82 us per RHS, the gather kernel 87 us).
@@ -27,8 +28,8 @@ const FormatShape kOneRhs[6] = {{PLQCD_PREC_F64, 18, {1, 1, 256, 32, 1, 4, 1}},
                                 {PLQCD_PREC_F64, 12, {1, 1, 256, 32, 1, 1, 4}},
                                 {PLQCD_PREC_F32, 18, {1, 1, 256, 32, 1, 8, 1}},
                                 {PLQCD_PREC_F32, 12, {2, 8, 256, 8, 1, 4, 4}},
-                                {PLQCD_PREC_F16, 12, {1, 1, 256, 4, 1, 0, 0}},
-                                {PLQCD_PREC_F16, 18, {2, 8, 512, 4, 1, 0, 0}}};
+                                {PLQCD_PREC_F16, 12, {1, 1, 512, 8, 1, 0, 0}},
+                                {PLQCD_PREC_F16, 18, {1, 1, 512, 8, 1, 0, 0}}};
 const HopTuning kMultiRhs[3] = {{2, 8, 256, 16, 1, 0, 0}, {2, 8, 256, 4, 1, 0, 0}, {2, 8, 256, 4, 1, 0, 0}};
 // Several RHS with the solver epilogues (M, and M^+ with the CG update): the
 // F16 gather kernel, whose 64-72 VGPRs leave room for more waves to overlap
