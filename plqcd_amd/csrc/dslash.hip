@@ -18,7 +18,9 @@
 // links for 4 t-slices).  Shapes that do not
 // fit a lattice fall back to the untiled t-march or the gather kernel.  Several
 // RHS per launch, per precision: XCDs sweeping the lattice together, with
-// short t-marches (F16 at 12 RHS: 82 us per RHS, the gather kernel 87 us).
+// t-marches of 8 slices for F32/F16 (F16 at 12 RHS: 80 us per RHS, the
+// gather kernel 87 us) and 16 for F64 (which divides 48 as well as 32 and 64;
+// longer ones gain under 1 % at 32^3 x 64).
 namespace {
 struct FormatShape {
    int prec, links;
@@ -30,7 +32,7 @@ const FormatShape kOneRhs[6] = {{PLQCD_PREC_F64, 18, {1, 1, 256, 32, 1, 4, 1}},
                                 {PLQCD_PREC_F32, 12, {2, 8, 256, 8, 1, 4, 4}},
                                 {PLQCD_PREC_F16, 12, {1, 1, 512, 8, 1, 0, 0}},
                                 {PLQCD_PREC_F16, 18, {1, 1, 512, 8, 1, 0, 0}}};
-const HopTuning kMultiRhs[3] = {{2, 8, 256, 16, 1, 0, 0}, {2, 8, 256, 4, 1, 0, 0}, {2, 8, 256, 4, 1, 0, 0}};
+const HopTuning kMultiRhs[3] = {{2, 8, 256, 16, 1, 0, 0}, {2, 8, 256, 8, 1, 0, 0}, {2, 8, 256, 8, 1, 0, 0}};
 // Several RHS with the solver epilogues (M, and M^+ with the CG update): the
 // F16 gather kernel, whose 64-72 VGPRs leave room for more waves to overlap
 // the epilogue's streams (12-RHS FP16 CG at 32^3 x 64: 0.264 s, with the
