@@ -239,37 +239,40 @@ def solver_suite():
             emit(rec)
     # ragged batch: 16^3x32, 24^3x48, 32^3x64, each 12 RHS, one persistent launch
     shapes = [(16, 16, 16, 32), (24, 24, 24, 48), (32, 32, 32, 64)]
-    lats, gs_, fis, fos = [], [], [], []
-    for k, Ls in enumerate(shapes):
-        Vs = int(np.prod(Ls))
-        la = hp.Lattice(Ls)
-        la.set_stream(lib.plqcd_hip_lattice_stream(lat.h))
-        lats.append(la)
-        gs_.append(hp.Gauge(la, 1, 12).upload(hp.random_gauge_su3(Vs, 5 + k)))
-        fi, fo = hp.Field(la, 1, 12), hp.Field(la, 1, 12)
-        for r in range(12):
-            fi.upload(hp.random_spinor(Vs, 50 + r), r)
-        fis.append(fi); fos.append(fo)
-    n = len(shapes)
-    garr = (C.c_void_p * n)(*[x_.h.value for x_ in gs_])
-    iarr = (C.c_void_p * n)(*[x_.h.value for x_ in fis])
-    oarr = (C.c_void_p * n)(*[x_.h.value for x_ in fos])
-    for _ in range(3):
-        capi.check(lib.plqcd_hip_hop_batch(n, garr, iarr, oarr, 1), "hop_batch")
-    lat.timer_start()
-    for _ in range(10):
-        capi.check(lib.plqcd_hip_hop_batch(n, garr, iarr, oarr, 1), "hop_batch")
-    t_batch = lat.timer_stop() / 10
-    lat.timer_start()
-    for _ in range(10):
+    sources = [[hp.random_spinor(int(np.prod(Ls)), 50 + r) for r in range(12)] for Ls in shapes]
+    for prec, links in ((1, 12), (2, 12), (0, 18)):
+        lats, gs_, fis, fos = [], [], [], []
+        for k, Ls in enumerate(shapes):
+            Vs = int(np.prod(Ls))
+            la = hp.Lattice(Ls)
+            la.set_stream(lib.plqcd_hip_lattice_stream(lat.h))
+            lats.append(la)
+            gs_.append(hp.Gauge(la, prec, links).upload(hp.random_gauge_su3(Vs, 5 + k)))
+            fi, fo = hp.Field(la, prec, 12), hp.Field(la, prec, 12)
+            for r in range(12):
+                fi.upload(sources[k][r], r)
+            fis.append(fi); fos.append(fo)
+        n = len(shapes)
+        garr = (C.c_void_p * n)(*[x_.h.value for x_ in gs_])
+        iarr = (C.c_void_p * n)(*[x_.h.value for x_ in fis])
+        oarr = (C.c_void_p * n)(*[x_.h.value for x_ in fos])
+        for _ in range(3):
+            capi.check(lib.plqcd_hip_hop_batch(n, garr, iarr, oarr, 1), "hop_batch")
+        lat.timer_start()
+        for _ in range(10):
+            capi.check(lib.plqcd_hip_hop_batch(n, garr, iarr, oarr, 1), "hop_batch")
+        t_batch = lat.timer_stop() / 10
+        lat.timer_start()
+        for _ in range(10):
+            for k in range(n):
+                hp.hop(gs_[k], fis[k], fos[k], hp.PARITY_ALL, hp.HOP_WILSON)
+        t_sep = lat.timer_stop() / 10
+        sites = sum(int(np.prod(Ls)) for Ls in shapes) * 12
+        emit({"suite": "solver", "case": f"ragged_batch_{PREC_NAMES[prec]}_l{links}_3lattices_12rhs",
+              "us_one_launch": round(t_batch * 1e3, 1), "us_separate_launches": round(t_sep * 1e3, 1),
+              "gflops_one_launch": round(FLOP_PER_SITE * sites / t_batch / 1e6, 1)})
         for k in range(n):
-            hp.hop(gs_[k], fis[k], fos[k], hp.PARITY_ALL, hp.HOP_WILSON)
-    t_sep = lat.timer_stop() / 10
-    sites = sum(int(np.prod(Ls)) for Ls in shapes) * 12
-    emit({"suite": "solver", "case": "ragged_batch_f32_l12_3lattices_12rhs", "us_one_launch": round(t_batch * 1e3, 1),
-          "us_separate_launches": round(t_sep * 1e3, 1), "gflops_one_launch": round(FLOP_PER_SITE * sites / t_batch / 1e6, 1)})
-    for k in range(n):
-        fis[k].close(); fos[k].close(); gs_[k].close(); lats[k].close()
+            fis[k].close(); fos[k].close(); gs_[k].close(); lats[k].close()
     # autotuner at 32^3 x 64
     for prec, links, R in ((0, 18, 1), (1, 12, 12), (2, 12, 12)):
         g = hp.Gauge(lat, prec, links).upload(u)

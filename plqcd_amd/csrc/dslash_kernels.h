@@ -717,7 +717,12 @@ __device__ __forceinline__ void hop_tmarch_block(const SIO &in, const SO &out, c
             typename SIO::Hv rr;
             const int tp = (t + 1 == Lt) ? 0 : t + 1;
             si.load_sp(p ^ 1, c + tp * S, f);
-            g.load_u(p, 3, j, u);
+            // U_t(t) serves this slice's forward and the next slice's
+            // backward t term from registers and no other block reads it: a
+            // streaming load (FP16 with one RHS 6-9 % faster at 32^3 x 64,
+            // 12 RHS under 1 % slower; choosing by R at run time cost the
+            // one-RHS kernel its gain)
+            g.template load_u<true>(p, 3, j, u);
             term_u<3, SF>(f, u, rr);
             accum<3, SF, true>(acc, rr);
             accum<3, -SF, true>(acc, b);
@@ -916,7 +921,7 @@ __global__ __launch_bounds__(BLOCK) void hop_tile_kernel(SIO in, SIO out, GIO g,
          in.load_sp(p ^ 1, c + tp * S, f);
 #pragma unroll
          for (int m = 0; m < Sp::N; m++) nxt[m][tid] = f.e[m];
-         g.load_u(p, 3, j, u);
+         g.template load_u<true>(p, 3, j, u);  // read by this lane only (see hop_tmarch_kernel)
          term_u<3, SF>(f, u, rr);
          accum<3, SF, true>(acc, rr);
 #pragma unroll

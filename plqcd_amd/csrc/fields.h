@@ -22,10 +22,15 @@
 // vector arithmetic.  (A parity block of one component of a field must stay
 // below 4 GiB: Vh*R*16 bytes.)
 
-template <class T>
+// NT: a non-temporal (streaming) load, for data that no other lane or block
+// reads again soon; it leaves more of the L2 to data that is re-read.
+template <class T, bool NT = false>
 __device__ __forceinline__ T ld(const char *ubase, uint32_t off)
 {
-   return *reinterpret_cast<const T *>(ubase + off);
+   if constexpr (NT)
+      return __builtin_nontemporal_load(reinterpret_cast<const T *>(ubase + off));
+   else
+      return *reinterpret_cast<const T *>(ubase + off);
 }
 template <class T>
 __device__ __forceinline__ void st_nt(char *ubase, uint32_t off, T v)
@@ -302,14 +307,19 @@ struct GaugeIO<PLQCD_PREC_F64, 18> {
    typedef LinkV<C> U;
    const char *base;
    size_t kstride;  // Vh*16: bytes between link components
+   template <bool NT = false>
    __device__ __forceinline__ void load(int parity, int mu, int j, C u[9]) const
    {
       const char *b = base + (size_t)(parity * 4 + mu) * 9 * kstride;
       const uint32_t o = (uint32_t)j * 16;
 #pragma unroll
-      for (int k = 0; k < 9; k++) u[k] = ld<d2>(b + k * kstride, o);
+      for (int k = 0; k < 9; k++) u[k] = ld<d2, NT>(b + k * kstride, o);
    }
-   __device__ __forceinline__ void load_u(int parity, int mu, int j, U &u) const { load(parity, mu, j, u.e); }
+   template <bool NT = false>
+   __device__ __forceinline__ void load_u(int parity, int mu, int j, U &u) const
+   {
+      load<NT>(parity, mu, j, u.e);
+   }
 };
 
 template <>
@@ -318,15 +328,20 @@ struct GaugeIO<PLQCD_PREC_F64, 12> {
    typedef LinkV<C> U;
    const char *base;
    size_t kstride;
+   template <bool NT = false>
    __device__ __forceinline__ void load(int parity, int mu, int j, C u[9]) const
    {
       const char *b = base + (size_t)(parity * 4 + mu) * 6 * kstride;
       const uint32_t o = (uint32_t)j * 16;
 #pragma unroll
-      for (int k = 0; k < 6; k++) u[k] = ld<d2>(b + k * kstride, o);
+      for (int k = 0; k < 6; k++) u[k] = ld<d2, NT>(b + k * kstride, o);
       reconstruct_row2(u);
    }
-   __device__ __forceinline__ void load_u(int parity, int mu, int j, U &u) const { load(parity, mu, j, u.e); }
+   template <bool NT = false>
+   __device__ __forceinline__ void load_u(int parity, int mu, int j, U &u) const
+   {
+      load<NT>(parity, mu, j, u.e);
+   }
 };
 
 template <>
@@ -335,14 +350,19 @@ struct GaugeIO<PLQCD_PREC_F32, 18> {
    typedef LinkV<C> U;
    const char *base;
    size_t kstride;  // Vh*8
+   template <bool NT = false>
    __device__ __forceinline__ void load(int parity, int mu, int j, C u[9]) const
    {
       const char *b = base + (size_t)(parity * 4 + mu) * 9 * kstride;
       const uint32_t o = (uint32_t)j * 8;
 #pragma unroll
-      for (int k = 0; k < 9; k++) u[k] = ld<f2>(b + k * kstride, o);
+      for (int k = 0; k < 9; k++) u[k] = ld<f2, NT>(b + k * kstride, o);
    }
-   __device__ __forceinline__ void load_u(int parity, int mu, int j, U &u) const { load(parity, mu, j, u.e); }
+   template <bool NT = false>
+   __device__ __forceinline__ void load_u(int parity, int mu, int j, U &u) const
+   {
+      load<NT>(parity, mu, j, u.e);
+   }
 };
 
 template <>
@@ -351,19 +371,24 @@ struct GaugeIO<PLQCD_PREC_F32, 12> {
    typedef LinkV<C> U;
    const char *base;
    size_t kstride;  // Vh*16
+   template <bool NT = false>
    __device__ __forceinline__ void load(int parity, int mu, int j, C u[9]) const
    {
       const char *b = base + (size_t)(parity * 4 + mu) * 3 * kstride;
       const uint32_t o = (uint32_t)j * 16;
 #pragma unroll
       for (int k = 0; k < 3; k++) {
-         const f4 v = ld<f4>(b + k * kstride, o);
+         const f4 v = ld<f4, NT>(b + k * kstride, o);
          u[2 * k] = f2{v.x, v.y};
          u[2 * k + 1] = f2{v.z, v.w};
       }
       reconstruct_row2(u);
    }
-   __device__ __forceinline__ void load_u(int parity, int mu, int j, U &u) const { load(parity, mu, j, u.e); }
+   template <bool NT = false>
+   __device__ __forceinline__ void load_u(int parity, int mu, int j, U &u) const
+   {
+      load<NT>(parity, mu, j, u.e);
+   }
 };
 
 // F16 links are held as half pairs, which the F16 hopping arithmetic multiplies
@@ -423,11 +448,12 @@ struct GaugeIO<PLQCD_PREC_F16, 12> {
    typedef LinkH U;
    const char *base;  // [parity][mu][Vh] u4 (u00,u01,u02,u10), then [parity][mu][Vh] u2 (u11,u12)
    size_t kstride;    // Vh*16
+   template <bool NT = false>
    __device__ __forceinline__ void load_u(int parity, int mu, int j, U &u) const
    {
       const size_t slot = (size_t)(parity * 4 + mu);
-      const u4 a = ld<u4>(base + slot * kstride, (uint32_t)j * 16);
-      const u2 b = ld<u2>(base + 8 * kstride + slot * (kstride / 2), (uint32_t)j * 8);
+      const u4 a = ld<u4, NT>(base + slot * kstride, (uint32_t)j * 16);
+      const u2 b = ld<u2, NT>(base + 8 * kstride + slot * (kstride / 2), (uint32_t)j * 8);
       u.e[0] = a.x;
       u.e[1] = a.y;
       u.e[2] = a.z;
@@ -444,11 +470,12 @@ struct GaugeIO<PLQCD_PREC_F16, 18> {
    typedef LinkH U;
    const char *base;  // [parity][mu][Vh] u4 (u00..u10), u4 (u11..u21), then [parity][mu][Vh] u32 (u22)
    size_t kstride;    // Vh*16
+   template <bool NT = false>
    __device__ __forceinline__ void load_u(int parity, int mu, int j, U &u) const
    {
       const size_t slot = (size_t)(parity * 4 + mu);
-      const u4 a = ld<u4>(base + slot * kstride, (uint32_t)j * 16);
-      const u4 b = ld<u4>(base + (8 + slot) * kstride, (uint32_t)j * 16);
+      const u4 a = ld<u4, NT>(base + slot * kstride, (uint32_t)j * 16);
+      const u4 b = ld<u4, NT>(base + (8 + slot) * kstride, (uint32_t)j * 16);
       u.e[0] = a.x;
       u.e[1] = a.y;
       u.e[2] = a.z;
@@ -457,7 +484,7 @@ struct GaugeIO<PLQCD_PREC_F16, 18> {
       u.e[5] = b.y;
       u.e[6] = b.z;
       u.e[7] = b.w;
-      u.e[8] = ld<uint32_t>(base + 16 * kstride + slot * (kstride / 4), (uint32_t)j * 4);
+      u.e[8] = ld<uint32_t, NT>(base + 16 * kstride + slot * (kstride / 4), (uint32_t)j * 4);
    }
 };
 
