@@ -224,7 +224,7 @@ typedef struct {
    int max_iter;            /* iterations of the inner CG                       */
    double delta;            /* reliable update when the iterated residual drops
                                by delta since the last one; <= 0: 1e-3 for F32,
-                               0.1 for F16 sloppy, only at convergence for F64  */
+                               1e-2 for F16 sloppy, only at convergence for F64 */
    int use_graph;           /* replay one captured hipGraph per iteration       */
    const double *kappa_rhs; /* optional per-RHS kappa (b->nrhs values)          */
    const double *tol_rhs;   /* optional per-RHS tolerance                       */
