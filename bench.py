@@ -190,7 +190,7 @@ def time_hop(hp, g, fi, fo, n=20):
 
 def solver_suite():
     """configs[2] (48^3x96 CG: FP64 vs mixed), configs[3] (multi-RHS hop and
-    CG at 32^3x64, ragged persistent batch, autotuner).  One JSON line each."""
+    CG at 32^3x64, ragged batch in one launch, autotuner).  One JSON line each."""
     import ctypes as C
     from plqcd_amd import capi, hopping as hp, solver as sv
     lib = capi.lib()
@@ -237,7 +237,7 @@ def solver_suite():
             rec = solve_compare(L, nrhs=R, sloppy=sl)
             rec.update({"suite": "solver", "case": f"cg_32c64_mixed_{PREC_NAMES[sl[0]]}_R{R}_vs_fp64_R1"})
             emit(rec)
-    # ragged batch: 16^3x32, 24^3x48, 32^3x64, each 12 RHS, one persistent launch
+    # ragged batch: 16^3x32, 24^3x48, 32^3x64, each 12 RHS, one launch
     shapes = [(16, 16, 16, 32), (24, 24, 24, 48), (32, 32, 32, 64)]
     sources = [[hp.random_spinor(int(np.prod(Ls)), 50 + r) for r in range(12)] for Ls in shapes]
     for prec, links in ((1, 12), (2, 12), (0, 18)):
@@ -258,15 +258,19 @@ def solver_suite():
         oarr = (C.c_void_p * n)(*[x_.h.value for x_ in fos])
         for _ in range(3):
             capi.check(lib.plqcd_hip_hop_batch(n, garr, iarr, oarr, 1), "hop_batch")
-        lat.timer_start()
-        for _ in range(10):
-            capi.check(lib.plqcd_hip_hop_batch(n, garr, iarr, oarr, 1), "hop_batch")
-        t_batch = lat.timer_stop() / 10
-        lat.timer_start()
-        for _ in range(10):
-            for k in range(n):
-                hp.hop(gs_[k], fis[k], fos[k], hp.PARITY_ALL, hp.HOP_WILSON)
-        t_sep = lat.timer_stop() / 10
+        # one launch and three launches alternate, three times; medians
+        tb, ts = [], []
+        for _ in range(3):
+            lat.timer_start()
+            for _ in range(20):
+                capi.check(lib.plqcd_hip_hop_batch(n, garr, iarr, oarr, 1), "hop_batch")
+            tb.append(lat.timer_stop() / 20)
+            lat.timer_start()
+            for _ in range(20):
+                for k in range(n):
+                    hp.hop(gs_[k], fis[k], fos[k], hp.PARITY_ALL, hp.HOP_WILSON)
+            ts.append(lat.timer_stop() / 20)
+        t_batch, t_sep = float(np.median(tb)), float(np.median(ts))
         sites = sum(int(np.prod(Ls)) for Ls in shapes) * 12
         emit({"suite": "solver", "case": f"ragged_batch_{PREC_NAMES[prec]}_l{links}_3lattices_12rhs",
               "us_one_launch": round(t_batch * 1e3, 1), "us_separate_launches": round(t_sep * 1e3, 1),

@@ -342,7 +342,7 @@ int plqcd_hip_autotune_clear(void);
 
 /* One application of H (full lattice, given mode) on each of n independent
  * lattices -- each with its own gauge field and fields, any volume, up to 32
- * RHS -- in one persistent launch per 16 lattices.  All items share storage
+ * RHS -- in one launch per 16 lattices.  All items share storage
  * precision, link format and stream.  Asynchronous. */
 int plqcd_hip_hop_batch(int n, const plqcd_hip_gauge *const *g, const plqcd_hip_field *const *in,
                         plqcd_hip_field *const *out, int mode);

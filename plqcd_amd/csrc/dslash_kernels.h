@@ -611,10 +611,8 @@ __global__ __launch_bounds__(BLOCK) void hop_rb_kernel(SIO in, SIO out, GIO g, H
 }
 
 // One application of H on each of several independent lattices (each with its
-// own gauge field and fields) in a single persistent launch: the grid holds as
-// many blocks as are resident at once, and block b walks the concatenated
-// logical blocks b, b + gridDim.x, ...  (every block leaves after a bounded
-// number of iterations).
+// own gauge field and fields) in a single launch over the concatenated
+// logical blocks of the items.
 template <class SIO, class GIO>
 struct BatchItem {
    SIO in, out;
@@ -627,6 +625,7 @@ struct BatchItem {
 // A launch takes up to kBatchItems items by value, as a kernel argument (no
 // device copy of the list); longer batches take several launches.
 constexpr int kBatchItems = 16;
+constexpr int kBatchGroup = 8;  // consecutive logical blocks an XCD takes in a ragged batch
 template <class SIO, class GIO>
 struct BatchList {
    BatchItem<SIO, GIO> it[kBatchItems];
@@ -656,7 +655,7 @@ __device__ __forceinline__ void hop_tmarch_block(const SIO &in, const SO &out, c
    const int r = threadIdx.x % a.R;
    const int js = threadIdx.x / a.R;
    const int c = sb * a.spb + js;
-   const bool valid = js < a.spb && c < S;  // a persistent batch block may have spare lanes
+   const bool valid = js < a.spb && c < S;  // a batch block may have spare lanes
    const SIO si = in.at_rhs(a.r0 + r);
    const SO so = out.at_rhs(a.r0 + r);
 
@@ -748,21 +747,22 @@ __global__ __launch_bounds__(BLOCK) void hop_tmarch_kernel(SIO in, SO out, GIO g
    hop_tmarch_block<DAG, EPI, NORM, BLOCK>(in, out, g, a, logical_block(a), fuse);
 }
 
-// The persistent launch over a ragged batch (see BatchItem): each item runs
-// t-march or gather blocks.  Blocks go round-robin to the 8 XCDs; as in
-// block order 2, the XCDs sweep the concatenated logical blocks together, XCD
-// k (blockIdx % 8 == k) taking the k-th run of kGroup consecutive blocks of
-// every 8 kGroup.  The grid is a multiple of 8.  KIND 0: every item runs gather blocks, 1:
+// One launch over a ragged batch (see BatchItem): each item runs t-march or
+// gather blocks.  Blocks go round-robin to the 8 XCDs; as in block order 2,
+// the XCDs sweep the concatenated logical blocks together, XCD k (blockIdx %
+// 8 == k) taking the k-th run of kBatchGroup consecutive blocks of every
+// 8 kBatchGroup.  The grid is a multiple of 8; a block takes logical blocks
+// at a stride of gridDim / 8 runs, so any such grid covers the batch (the
+// launch gives each logical block a block of its own).  KIND 0: every item runs gather blocks, 1:
 // every item t-march blocks, 2: each item its own kind (the kernel then holds
 // both code paths and needs the registers of the larger).
 template <int MODE, int KIND, int BLOCK, class SIO, class GIO>
 __global__ __launch_bounds__(BLOCK) void hop_batch_kernel(const BatchList<SIO, GIO> items, int nitems, int total)
 {
-   constexpr int kGroup = 8;
    const int xcd = blockIdx.x & 7, per = gridDim.x >> 3;
-   const int span = ((total + 8 * kGroup - 1) / (8 * kGroup)) * kGroup;  // blocks per XCD, rounded up
+   const int span = ((total + 8 * kBatchGroup - 1) / (8 * kBatchGroup)) * kBatchGroup;  // blocks per XCD, rounded up
    for (int i = blockIdx.x >> 3; i < span; i += per) {
-      const int b = ((i / kGroup) * 8 + xcd) * kGroup + i % kGroup;
+      const int b = ((i / kBatchGroup) * 8 + xcd) * kBatchGroup + i % kBatchGroup;
       if (b >= total) break;
       int it = 0;
       while (it + 1 < nitems && b >= items.it[it + 1].first) it++;
@@ -1120,9 +1120,6 @@ int launch_batch_t(int n, const plqcd_hip_gauge *const *g, const plqcd_hip_field
    for (int i = 0; i < n; i++)
       if (in[i]->nrhs * REDUCE_CHUNK > 256) return plqcd::set_error(2, "hop_batch: at most 32 right-hand sides per item");
    hipStream_t st = g[0]->lat->stream;
-   int dev = 0, ncu = 0;
-   (void)hipGetDevice(&dev);
-   (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
    for (int i0 = 0; i0 < n; i0 += kBatchItems) {
       const int m = std::min(kBatchItems, n - i0);
       // block size: the largest whole number of reduction chunks of sites
@@ -1155,11 +1152,14 @@ int launch_batch_t(int n, const plqcd_hip_gauge *const *g, const plqcd_hip_field
          total += I.nb;
          nmarch += I.march;
       }
+      // one hardware block per logical block (8 XCDs x span, see
+      // hop_batch_kernel): the hardware then balances the items' blocks over
+      // the CUs.  A persistent grid of occupancy x CUs blocks, each taking a
+      // fixed stride of logical blocks, was 2-9 % slower than this at
+      // 16^3x32 + 24^3x48 + 32^3x64 with 12 RHS.
       auto go = [&](auto kern) {
-         int per = 0;
-         (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, block, 0);
-         const int grid = std::max(8, std::min(total, std::max(per, 1) * ncu) / 8 * 8);
-         hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, st, L, m, total);
+         const int span = ((total + 8 * kBatchGroup - 1) / (8 * kBatchGroup)) * kBatchGroup;
+         hipLaunchKernelGGL(kern, dim3(8 * span), dim3(block), 0, st, L, m, total);
       };
       if (mode == PLQCD_HOP_COMPAT) go(hop_batch_kernel<PLQCD_HOP_COMPAT, 0, 256, S, G>);
       else if (nmarch == 0) go(hop_batch_kernel<PLQCD_HOP_WILSON, 0, 256, S, G>);

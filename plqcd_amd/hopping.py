@@ -299,7 +299,7 @@ def hop(gauge: Gauge, fin: Field, fout: Field, parity: int = PARITY_ALL, mode: i
 
 def hop_batch(gauges, fins, fouts, mode: int = HOP_WILSON) -> None:
     """fouts[k] = H fins[k] (full lattice) for independent lattices in one
-    persistent launch; all items share one storage format and one stream."""
+    launch; all items share one storage format and one stream."""
     n = len(gauges)
     if not (n == len(fins) == len(fouts)) or n == 0:
         raise ValueError("hop_batch: gauges, fins and fouts must be non-empty and of equal length")

@@ -364,7 +364,7 @@ RAGGED = [[((4, 4, 4, 8), 3), ((8, 4, 6, 6), 1), ((8, 8, 8, 16), 4), ((4, 4, 4, 
 @pytest.mark.parametrize("shapes", RAGGED)
 @pytest.mark.parametrize("prec,links,tol", [(0, 18, TOL64), (1, 12, TOL32), (2, 12, TOL16), (2, 18, TOL16)])
 def test_ragged_batch_matches_separate_and_oracle(hp, orc, shapes, prec, links, tol):
-    """One persistent launch over lattices of different volumes and RHS counts
+    """One launch over lattices of different volumes and RHS counts
     (t-march items and items whose time extent takes gather blocks, mixed or
     all of one kind) gives each item bitwise its own launch's output, which
     matches the oracle."""
