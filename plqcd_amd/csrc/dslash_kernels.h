@@ -1047,8 +1047,8 @@ int launch_t(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip_fiel
    // extent is a multiple of T (and, with a norm, whose t-slices hold whole
    // reduction chunks)
    // (not with the residual epilogue: at 32^3 x 64 and 12 RHS that t-march
-   // read 23 GB per launch instead of about 18 GB and took 13 ms, the gather
-   // kernel 4-5 ms)
+   // holds 364 registers, 108 of them AGPRs, and took 13 ms, the gather
+   // kernel 7.2 ms)
    const bool march = !resid && rb == 1 && T > 0 && mode == PLQCD_HOP_WILSON && a.nparity == 2 &&
                       lat->L[3] % T == 0 && (!norm || a.S % REDUCE_CHUNK == 0);
    // the LDS-tiled t-march: one RHS, a tile of whole x rows dividing the y and
