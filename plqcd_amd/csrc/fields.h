@@ -297,6 +297,10 @@ inline SpinorIO<PREC> spinor_io(const plqcd_hip_field *f)
 }
 
 // -------------------------------------------------------------------- links
+//
+// load_u(parity, mu, j, u) reads the link U_mu of site j of `parity` into its
+// held form; load_u<true> does the same with streaming (non-temporal) loads,
+// for a link that only the loading lane reads.
 
 template <int PREC, int LINKS>
 struct GaugeIO;
