@@ -782,13 +782,18 @@ __global__ __launch_bounds__(BLOCK) void hop_batch_kernel(const BatchList<SIO, G
 
 // One direction of the tiled kernel: neighbours inside the tile come from the
 // LDS slots su / sd (>= 0), the others from memory; same operation sequence
-// as hop_fwd + hop_bwd.
-template <int MU, bool DAG, int BLOCK, class SIO, class GIO>
+// as hop_fwd + hop_bwd.  With XSH (x direction only) the lane's forward link
+// is streamed from memory and published in ux, and the backward link -- the
+// forward link of the x - 1 lane, slot sd -- is read from there after a
+// barrier; every lane of the block calls this.
+template <int MU, bool DAG, int BLOCK, bool XSH, class SIO, class GIO>
 __device__ __forceinline__ void tile_dir(const SIO &in, const GIO &g, const typename SIO::Sp::E (*buf)[BLOCK], int p,
-                                         int j, int base, int cu, int cd, int su, int sd, typename SIO::C acc[12])
+                                         int j, int base, int cu, int cd, int su, int sd, typename SIO::C acc[12],
+                                         typename GIO::U::E (*ux)[BLOCK])
 {
    typedef typename SIO::Sp Sp;
    constexpr int SG = DAG ? 1 : -1;
+   constexpr bool share = MU == 0 && XSH;
    Sp f;
    typename GIO::U u;
    typename SIO::Hv r;
@@ -798,7 +803,13 @@ __device__ __forceinline__ void tile_dir(const SIO &in, const GIO &g, const type
    } else {
       in.load_sp(1 - p, cu + base, f);
    }
-   g.load_u(p, MU, j, u);
+   if constexpr (share) {
+      g.template load_u<true>(p, MU, j, u);
+#pragma unroll
+      for (int k = 0; k < 9; k++) ux[k][threadIdx.x] = u.e[k];
+   } else {
+      g.load_u(p, MU, j, u);
+   }
    term_u<MU, SG>(f, u, r);
    accum<MU, SG, true>(acc, r);
    if (sd >= 0) {
@@ -807,7 +818,13 @@ __device__ __forceinline__ void tile_dir(const SIO &in, const GIO &g, const type
    } else {
       in.load_sp(1 - p, cd + base, f);
    }
-   g.load_u(1 - p, MU, cd + base, u);
+   if constexpr (share) {
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < 9; k++) u.e[k] = ux[k][sd];
+   } else {
+      g.load_u(1 - p, MU, cd + base, u);
+   }
    term_udag<MU, -SG>(f, u, r);
    accum<MU, -SG, true>(acc, r);
 }
@@ -820,8 +837,9 @@ __device__ __forceinline__ void tile_dir(const SIO &in, const GIO &g, const type
 // neighbours inside the tile are read from there; only the tile's y/z halo and
 // the t+1 spinor are loaded from memory.  The centre spinor and the backward
 // t term stay in LDS rather than in registers.  Links and the t direction are
-// handled as in
-// hop_tmarch_kernel, and terms are accumulated in the order of hop_kernel.
+// handled as in hop_tmarch_kernel, except that blocks of up to 128 lanes pass
+// the x links between the two lanes that use each one through LDS (a second
+// barrier per slice).  Terms are accumulated in the order of hop_kernel.
 // BLOCK = 2 Lxh ty tz lanes.
 template <bool DAG, int EPI, bool NORM, int BLOCK, class SIO, class GIO>
 __global__ __launch_bounds__(BLOCK) void hop_tile_kernel(SIO in, SIO out, GIO g, HopArgs a, CgFuse<SIO> fuse)
@@ -834,6 +852,13 @@ __global__ __launch_bounds__(BLOCK) void hop_tile_kernel(SIO in, SIO out, GIO g,
    typedef typename SIO::Hv Hv;
    __shared__ typename Sp::E sm[2][Sp::N][BLOCK];
    __shared__ typename Hv::E sb[Hv::N][BLOCK];  // the lane's backward t term b (read and written by that lane only)
+   // x links through LDS in blocks of two waves: a CU then holds several
+   // blocks, and one works while another waits at the extra barrier (FP64
+   // 18-real, 32x4x1 tiles at 32^3 x 64: 407 against 427 us; 256- and 512-lane
+   // FP32 tiles ran 1-19 % slower with it)
+   constexpr bool XSH = BLOCK <= 128;
+   __shared__ typename GIO::U::E ux[XSH ? 9 : 1][XSH ? BLOCK : 1];  // the lanes' forward x links of this slice
+   typedef typename GIO::U::E(*UxRow)[BLOCK];
    const int S = a.S;
    constexpr int ncol = BLOCK / 2;
    const int nty = a.L[1] / a.ty;
@@ -907,11 +932,11 @@ __global__ __launch_bounds__(BLOCK) void hop_tile_kernel(SIO in, SIO out, GIO g,
       for (int m = 0; m < 12; m++) acc[m] = C{0, 0};
       // the scheduling barriers keep each direction's loads in its own
       // section, which holds the kernel within 256 VGPRs without spills
-      tile_dir<0, DAG, BLOCK>(in, g, buf, p, j, base, cu[0], cd[0], su[0], sd[0], acc);
+      tile_dir<0, DAG, BLOCK, XSH>(in, g, buf, p, j, base, cu[0], cd[0], su[0], sd[0], acc, (UxRow)ux);
       __builtin_amdgcn_sched_barrier(0);
-      tile_dir<1, DAG, BLOCK>(in, g, buf, p, j, base, cu[1], cd[1], su[1], sd[1], acc);
+      tile_dir<1, DAG, BLOCK, XSH>(in, g, buf, p, j, base, cu[1], cd[1], su[1], sd[1], acc, (UxRow)ux);
       __builtin_amdgcn_sched_barrier(0);
-      tile_dir<2, DAG, BLOCK>(in, g, buf, p, j, base, cu[2], cd[2], su[2], sd[2], acc);
+      tile_dir<2, DAG, BLOCK, XSH>(in, g, buf, p, j, base, cu[2], cd[2], su[2], sd[2], acc, (UxRow)ux);
       __builtin_amdgcn_sched_barrier(0);
       {
          Sp f;

@@ -65,9 +65,11 @@ struct SpinH {
 };
 template <class C>
 struct LinkV {
+   typedef C E;
    C e[9];
 };
 struct LinkH {
+   typedef uint32_t E;
    uint32_t e[9];
    __device__ __forceinline__ hh2 h(int k) const { return __builtin_bit_cast(hh2, e[k]); }
 };
