@@ -837,9 +837,9 @@ __device__ __forceinline__ void tile_dir(const SIO &in, const GIO &g, const type
 // neighbours inside the tile are read from there; only the tile's y/z halo and
 // the t+1 spinor are loaded from memory.  The centre spinor and the backward
 // t term stay in LDS rather than in registers.  Links and the t direction are
-// handled as in hop_tmarch_kernel, except that blocks of up to 128 lanes pass
-// the x links between the two lanes that use each one through LDS (a second
-// barrier per slice).  Terms are accumulated in the order of hop_kernel.
+// handled as in hop_tmarch_kernel, except that FP64 blocks of up to 128 lanes
+// pass the x links between the two lanes that use each one through LDS (a
+// second barrier per slice).  Terms are accumulated in the order of hop_kernel.
 // BLOCK = 2 Lxh ty tz lanes.
 template <bool DAG, int EPI, bool NORM, int BLOCK, class SIO, class GIO>
 __global__ __launch_bounds__(BLOCK) void hop_tile_kernel(SIO in, SIO out, GIO g, HopArgs a, CgFuse<SIO> fuse)
@@ -852,11 +852,11 @@ __global__ __launch_bounds__(BLOCK) void hop_tile_kernel(SIO in, SIO out, GIO g,
    typedef typename SIO::Hv Hv;
    __shared__ typename Sp::E sm[2][Sp::N][BLOCK];
    __shared__ typename Hv::E sb[Hv::N][BLOCK];  // the lane's backward t term b (read and written by that lane only)
-   // x links through LDS in blocks of two waves: a CU then holds several
+   // x links through LDS in FP64 blocks of two waves: a CU then holds several
    // blocks, and one works while another waits at the extra barrier (FP64
-   // 18-real, 32x4x1 tiles at 32^3 x 64: 407 against 427 us; 256- and 512-lane
-   // FP32 tiles ran 1-19 % slower with it)
-   constexpr bool XSH = BLOCK <= 128;
+   // 18-real, 32x4x1 tiles at 32^3 x 64: 407 against 427 us).  FP32 tiles ran
+   // up to 27 % slower with it (12-real, 32x4x1, 16 slices: 220 against 173 us).
+   constexpr bool XSH = BLOCK <= 128 && std::is_same<typename SIO::real, double>::value;
    __shared__ typename GIO::U::E ux[XSH ? 9 : 1][XSH ? BLOCK : 1];  // the lanes' forward x links of this slice
    typedef typename GIO::U::E(*UxRow)[BLOCK];
    const int S = a.S;
