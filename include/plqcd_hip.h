@@ -160,7 +160,8 @@ typedef struct plqcd_hip_lattice plqcd_hip_lattice;
 typedef struct plqcd_hip_gauge plqcd_hip_gauge;
 typedef struct plqcd_hip_field plqcd_hip_field;
 
-/* A lattice with extents L[0..3] (x,y,z,t), each even and >= 2, on the
+/* A lattice with extents L[0..3] (x,y,z,t), each even and >= 2, with at most
+ * 2^29 sites (the kernels' 32-bit link offsets), on the
  * current HIP device, with its own stream.  All calls taking a lattice
  * object are asynchronous on that stream unless stated. */
 int plqcd_hip_lattice_create(const int L[4], plqcd_hip_lattice **out);
@@ -185,7 +186,14 @@ int plqcd_hip_gauge_upload(plqcd_hip_gauge *g, const su3 *u);
 /* Device bytes of the gauge field. */
 size_t plqcd_hip_gauge_bytes(const plqcd_hip_gauge *g);
 
+/* A field of nrhs right-hand sides (1 <= nrhs <= plqcd_hip_max_rhs(L)). */
 int plqcd_hip_field_create(plqcd_hip_lattice *lat, int prec, int nrhs, plqcd_hip_field **out);
+/* The largest RHS count of one field on a lattice with extents L: the
+ * kernels address a site inside a component block with a 32-bit byte offset,
+ * so V/2 * nrhs * 16 bytes must not exceed 4 GiB (and nrhs <= 4096).  0 for
+ * extents plqcd_hip_lattice_create rejects (odd, < 2, or V > 2^29).  Host
+ * only; needs no device. */
+int plqcd_hip_max_rhs(const int L[4]);
 void plqcd_hip_field_destroy(plqcd_hip_field *f);
 /* V spinors in the reference layout, even-odd order, for right-hand side
  * `rhs`.  Synchronous with respect to the host buffer. */
@@ -337,6 +345,14 @@ int plqcd_hip_hop_get_tuning(int prec, int out[7]);
  * application.  in/out are overwritten. */
 int plqcd_hip_autotune(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip_field *out, int reps,
                        int best[7], double *best_ms);
+/* The launch shape cached for the shape of (g, in) -- lattice, precision,
+ * link format, RHS count -- and operation `op`: 0 plain applications of H
+ * (what plqcd_hip_autotune tunes), 1 the solvers' M / M^+ launches (which
+ * keep their built-in shapes).  Returns 1 and fills shape = {order, group,
+ * block, tmarch, rb, tile_y, tile_z} when one is cached, 0 when none is, -1
+ * on bad arguments.  No counterpart in the reference, which chooses among its
+ * variants by hand (tests/test_hopping.c:283-560). */
+int plqcd_hip_autotune_lookup(const plqcd_hip_gauge *g, const plqcd_hip_field *in, int op, int shape[7]);
 /* Forgets every autotuned shape (back to the per-precision defaults). */
 int plqcd_hip_autotune_clear(void);
 

@@ -73,6 +73,7 @@ _PROTOS = {
     "plqcd_hip_cache_gauge": (None, [C.c_int]),
     "plqcd_hip_gauge_changed": (None, []),
     "plqcd_hip_lattice_create": (C.c_int, [i4, C.POINTER(vp)]),
+    "plqcd_hip_max_rhs": (C.c_int, [i4]),
     "plqcd_hip_lattice_destroy": (None, [vp]),
     "plqcd_hip_lattice_set_stream": (C.c_int, [vp, vp]),
     "plqcd_hip_lattice_stream": (vp, [vp]),
@@ -100,6 +101,7 @@ _PROTOS = {
     "plqcd_hip_server_destroy": (None, [vp]),
     "plqcd_hip_autotune": (C.c_int, [vp, vp, vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     "plqcd_hip_autotune_clear": (C.c_int, []),
+    "plqcd_hip_autotune_lookup": (C.c_int, [vp, vp, C.c_int, C.POINTER(C.c_int)]),
     "plqcd_hip_hop_batch": (C.c_int, [C.c_int, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.c_int]),
     "plqcd_hip_hop_tuning": (C.c_int, [C.c_int] * 8),
     "plqcd_hip_hop_get_tuning": (C.c_int, [C.c_int, C.POINTER(C.c_int)]),

@@ -141,7 +141,10 @@ extern "C" int plqcd_hip_lattice_create(const int L[4], plqcd_hip_lattice **out)
          return plqcd::set_error(2, "plqcd_hip_lattice_create: L[%d] = %d must be even and >= 2", mu, L[mu]);
       V *= L[mu];
    }
-   if (V > (1LL << 30)) return plqcd::set_error(2, "plqcd_hip_lattice_create: volume %lld too large", V);
+   // a link component block of Vh 16-byte entries must stay within the
+   // kernels' 32-bit offsets: V/2 * 16 <= 4 GiB
+   if (V > (1LL << 29))
+      return plqcd::set_error(2, "plqcd_hip_lattice_create: volume %lld exceeds 2^29 sites", V);
    plqcd_hip_lattice *lat = new (std::nothrow) plqcd_hip_lattice();
    if (!lat) return plqcd::set_error(4, "out of host memory");
    for (int mu = 0; mu < 4; mu++) lat->L[mu] = L[mu];
@@ -271,6 +274,11 @@ extern "C" int plqcd_hip_field_create(plqcd_hip_lattice *lat, int prec, int nrhs
    if (prec < PLQCD_PREC_F64 || prec > PLQCD_PREC_F16)
       return plqcd::set_error(2, "plqcd_hip_field_create: bad precision %d", prec);
    if (nrhs < 1 || nrhs > 4096) return plqcd::set_error(2, "plqcd_hip_field_create: bad RHS count %d", nrhs);
+   if (nrhs > plqcd::max_rhs_per_field(lat->Vh))
+      return plqcd::set_error(2,
+                              "plqcd_hip_field_create: %d right-hand sides of %d sites per parity exceed the "
+                              "4 GiB component block of the kernels' 32-bit offsets (at most %d)",
+                              nrhs, lat->Vh, plqcd::max_rhs_per_field(lat->Vh));
    plqcd_hip_field *f = new (std::nothrow) plqcd_hip_field();
    if (!f) return plqcd::set_error(4, "out of host memory");
    f->lat = lat;
@@ -284,6 +292,18 @@ extern "C" int plqcd_hip_field_create(plqcd_hip_lattice *lat, int prec, int nrhs
    }
    *out = f;
    return 0;
+}
+
+extern "C" int plqcd_hip_max_rhs(const int L[4])
+{
+   if (!L) return 0;
+   long long V = 1;
+   for (int mu = 0; mu < 4; mu++) {
+      if (L[mu] < 2 || (L[mu] & 1)) return 0;
+      V *= L[mu];
+   }
+   if (V > (1LL << 29)) return 0;
+   return plqcd::max_rhs_per_field((int)(V / 2));
 }
 
 extern "C" void plqcd_hip_field_destroy(plqcd_hip_field *f)

@@ -71,6 +71,16 @@ inline size_t spinor_parity_bytes(int prec, int Vh)
       default: return size_t(Vh) * (3 * 16 + 4);
    }
 }
+// The kernels address a site inside one component block of a field with a
+// 32-bit byte offset (j*R + r)*16, and a link with j*16 (fields.h), so a
+// component block of Vh*R 16-byte chunks must not exceed 4 GiB.  The largest
+// RHS count a field of this lattice may hold:
+constexpr uint64_t kOffsetLimit = uint64_t(1) << 32;
+inline int max_rhs_per_field(int Vh)
+{
+   const uint64_t n = kOffsetLimit / (uint64_t(Vh) * 16);
+   return n > 4096 ? 4096 : (int)n;
+}
 inline size_t gauge_bytes(int prec, int links, int Vh)
 {
    const size_t n = size_t(2) * 4 * Vh;  // links

@@ -81,19 +81,25 @@ template <int PREC, int LINKS>
 int hop_batch_launch(int n, const plqcd_hip_gauge *const *g, const plqcd_hip_field *const *in,
                      plqcd_hip_field *const *out, int mode);
 
-// Launch shapes found by the autotuner, per (lattice, precision, links, RHS).
+// Launch shapes found by the autotuner, per (lattice, precision, links, RHS,
+// operation).  `op` separates plain applications of H (0), which the
+// autotuner times, from the solver's M / M^+ launches with their epilogues
+// (1), whose built-in shapes differ and which a tuning of H must not change.
 struct TuneKey {
    int L[4];
    int prec, links, R;
+   int op;
    bool operator<(const TuneKey &o) const
    {
       for (int i = 0; i < 4; i++)
          if (L[i] != o.L[i]) return L[i] < o.L[i];
       if (prec != o.prec) return prec < o.prec;
       if (links != o.links) return links < o.links;
-      return R < o.R;
+      if (R != o.R) return R < o.R;
+      return op < o.op;
    }
 };
+constexpr int kTuneOpHop = 0, kTuneOpSolver = 1;
 bool lookup_tuning(const TuneKey &k, HopTuning *t);
 void store_tuning(const TuneKey &k, const HopTuning &t);
 void clear_tuning(const TuneKey *k);  // null: all

@@ -780,6 +780,21 @@ __global__ __launch_bounds__(BLOCK) void hop_batch_kernel(const BatchList<SIO, G
    }
 }
 
+// LDS bytes of the tiled kernel for BLOCK lanes: two buffers of centre
+// spinors, the backward t terms and, in FP64 blocks of up to 128 lanes, the
+// forward x links of a slice (hop_tile_kernel).
+template <class S, class G, int BLOCK>
+constexpr bool tile_xsh()
+{
+   return BLOCK <= 128 && std::is_same<typename S::real, double>::value;
+}
+template <class S, class G, int BLOCK>
+constexpr size_t tile_lds_bytes()
+{
+   return (2 * sizeof(typename S::Sp) + sizeof(typename S::Hv)) * BLOCK +
+          (tile_xsh<S, G, BLOCK>() ? 9 * sizeof(typename G::U::E) * BLOCK : 0);
+}
+
 // One direction of the tiled kernel: neighbours inside the tile come from the
 // LDS slots su / sd (>= 0), the others from memory; same operation sequence
 // as hop_fwd + hop_bwd.  With XSH (x direction only) the lane's forward link
@@ -856,7 +871,7 @@ __global__ __launch_bounds__(BLOCK) void hop_tile_kernel(SIO in, SIO out, GIO g,
    // blocks, and one works while another waits at the extra barrier (FP64
    // 18-real, 32x4x1 tiles at 32^3 x 64: 407 against 427 us).  FP32 tiles ran
    // up to 27 % slower with it (12-real, 32x4x1, 16 slices: 220 against 173 us).
-   constexpr bool XSH = BLOCK <= 128 && std::is_same<typename SIO::real, double>::value;
+   constexpr bool XSH = tile_xsh<SIO, GIO, BLOCK>();
    __shared__ typename GIO::U::E ux[XSH ? 9 : 1][XSH ? BLOCK : 1];  // the lanes' forward x links of this slice
    typedef typename GIO::U::E(*UxRow)[BLOCK];
    const int S = a.S;
@@ -979,38 +994,52 @@ __global__ __launch_bounds__(BLOCK) void hop_tile_kernel(SIO in, SIO out, GIO g,
 enum Variant { V_GATHER = 0, V_MARCH = 1, V_RB2 = 2, V_RB4 = 4, V_TILE = 8 };
 
 // S: the input field's accessor; SO: the output's (differs only with EPI 3,
-// which runs the gather kernel).
+// which runs the gather kernel).  Returns 0 when the kernel was enqueued, and
+// an error when the variant does not exist for these template arguments (the
+// host-side checks of launch_t keep such launches out; this keeps a future
+// change from silently launching nothing).
 template <int MODE, bool DAG, int EPI, bool NORM, int BLOCK, class S, class G, class SO>
-void launch_one(int variant, dim3 grid, dim3 block, hipStream_t st, S si, SO so, G gg, const HopArgs &a,
-                const CgFuse<S> &f)
+int launch_one(int variant, dim3 grid, dim3 block, hipStream_t st, S si, SO so, G gg, const HopArgs &a,
+               const CgFuse<S> &f)
 {
    constexpr bool same = std::is_same<S, SO>::value && EPI != 3;  // the tile and RHS-blocked kernels
    if (variant == V_MARCH) {
-      if constexpr (MODE == PLQCD_HOP_WILSON && EPI != 3)
+      if constexpr (MODE == PLQCD_HOP_WILSON && EPI != 3) {
          hipLaunchKernelGGL((hop_tmarch_kernel<DAG, EPI, NORM, BLOCK, S, G, SO>), grid, block, 0, st, si, so, gg, a, f);
+         return 0;
+      }
    } else if (variant == V_TILE) {
-      // two LDS buffers of BLOCK spinors and BLOCK half-spinors must fit the CU's 160 KiB
-      if constexpr (same && MODE == PLQCD_HOP_WILSON &&
-                    (2 * sizeof(typename S::Sp) + sizeof(typename S::Hv)) * BLOCK <= 136 * 1024)
+      // two LDS buffers of BLOCK spinors, BLOCK half-spinors and (FP64 blocks
+      // of up to 128 lanes) BLOCK x links must fit the CU's 160 KiB
+      if constexpr (same && MODE == PLQCD_HOP_WILSON && tile_lds_bytes<S, G, BLOCK>() <= 136 * 1024) {
          hipLaunchKernelGGL((hop_tile_kernel<DAG, EPI, NORM, BLOCK, S, G>), grid, block, 0, st, si, so, gg, a, f);
+         return 0;
+      }
    } else if (variant == V_RB2) {
-      if constexpr (same)
+      if constexpr (same) {
          hipLaunchKernelGGL((hop_rb_kernel<MODE, DAG, EPI, NORM, BLOCK, 2, S, G>), grid, block, 0, st, si, so, gg, a, f);
+         return 0;
+      }
    } else if (variant == V_RB4) {
-      if constexpr (same)
+      if constexpr (same) {
          hipLaunchKernelGGL((hop_rb_kernel<MODE, DAG, EPI, NORM, BLOCK, 4, S, G>), grid, block, 0, st, si, so, gg, a, f);
-   } else {
+         return 0;
+      }
+   } else if (variant == V_GATHER) {
       hipLaunchKernelGGL((hop_kernel<MODE, DAG, EPI, NORM, BLOCK, S, G, SO>), grid, block, 0, st, si, so, gg, a, f);
+      return 0;
    }
+   return plqcd::set_error(2, "hop: kernel variant %d with %d-lane blocks does not exist for this operation", variant,
+                           BLOCK);
 }
 
 template <int MODE, bool DAG, int EPI, bool NORM, class S, class G, class SO>
-void launch_block(int maxblock, int variant, dim3 grid, dim3 block, hipStream_t st, S si, SO so, G gg,
-                  const HopArgs &a, const CgFuse<S> &f = CgFuse<S>{})
+int launch_block(int maxblock, int variant, dim3 grid, dim3 block, hipStream_t st, S si, SO so, G gg,
+                 const HopArgs &a, const CgFuse<S> &f = CgFuse<S>{})
 {
-   if (maxblock <= 128) launch_one<MODE, DAG, EPI, NORM, 128>(variant, grid, block, st, si, so, gg, a, f);
-   else if (maxblock <= 256) launch_one<MODE, DAG, EPI, NORM, 256>(variant, grid, block, st, si, so, gg, a, f);
-   else launch_one<MODE, DAG, EPI, NORM, 512>(variant, grid, block, st, si, so, gg, a, f);
+   if (maxblock <= 128) return launch_one<MODE, DAG, EPI, NORM, 128>(variant, grid, block, st, si, so, gg, a, f);
+   if (maxblock <= 256) return launch_one<MODE, DAG, EPI, NORM, 256>(variant, grid, block, st, si, so, gg, a, f);
+   return launch_one<MODE, DAG, EPI, NORM, 512>(variant, grid, block, st, si, so, gg, a, f);
 }
 
 template <int PREC, int LINKS>
@@ -1027,8 +1056,11 @@ int launch_t(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip_fiel
    // lanes: the window of right-hand sides [op.r0, op.r0 + R) of the fields
    const int R = op.nr > 0 ? op.nr : in->nrhs;
    a.r0 = op.nr > 0 ? op.r0 : 0;
-   HopTuning tu = plqcd::default_tuning(PREC, LINKS, R, op.kappa != nullptr);
-   plqcd::lookup_tuning(plqcd::TuneKey{{lat->L[0], lat->L[1], lat->L[2], lat->L[3]}, PREC, LINKS, R}, &tu);
+   const bool solver = op.kappa != nullptr;
+   HopTuning tu = plqcd::default_tuning(PREC, LINKS, R, solver);
+   plqcd::lookup_tuning(plqcd::TuneKey{{lat->L[0], lat->L[1], lat->L[2], lat->L[3]}, PREC, LINKS, R,
+                                       solver ? plqcd::kTuneOpSolver : plqcd::kTuneOpHop},
+                        &tu);
    // RHS per lane (RHS-blocked gather) when it divides R
    // (the residual epilogue runs the gather kernel only)
    const int rb = !resid && (tu.rb == 2 || tu.rb == 4) && R % tu.rb == 0 ? tu.rb : 1;
@@ -1102,34 +1134,38 @@ int launch_t(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip_fiel
    if ((a.order == 1 && nb % 8) || (a.order == 2 && nb % (8 * a.group))) a.order = 0;
    const dim3 grid(nb), block(nthreads);
    hipStream_t st = lat->stream;
+   int lrc = 0;
    if (epi == 3) {
       if constexpr (PREC == PLQCD_PREC_F64) {
          if (out->prec == PLQCD_PREC_F64)
-            launch_block<PLQCD_HOP_WILSON, true, 3, true>(mblock, variant, grid, block, st, si,
-                                                           spinor_io<PLQCD_PREC_F64>(out), gg, a, fu);
+            lrc = launch_block<PLQCD_HOP_WILSON, true, 3, true>(mblock, variant, grid, block, st, si,
+                                                                 spinor_io<PLQCD_PREC_F64>(out), gg, a, fu);
          else if (out->prec == PLQCD_PREC_F32)
-            launch_block<PLQCD_HOP_WILSON, true, 3, true>(mblock, variant, grid, block, st, si,
-                                                           spinor_io<PLQCD_PREC_F32>(out), gg, a, fu);
+            lrc = launch_block<PLQCD_HOP_WILSON, true, 3, true>(mblock, variant, grid, block, st, si,
+                                                                 spinor_io<PLQCD_PREC_F32>(out), gg, a, fu);
          else
-            launch_block<PLQCD_HOP_WILSON, true, 3, true>(mblock, variant, grid, block, st, si,
-                                                           spinor_io<PLQCD_PREC_F16>(out), gg, a, fu);
+            lrc = launch_block<PLQCD_HOP_WILSON, true, 3, true>(mblock, variant, grid, block, st, si,
+                                                                 spinor_io<PLQCD_PREC_F16>(out), gg, a, fu);
       }
+      if (lrc) return lrc;
       PLQCD_HIP_TRY(hipGetLastError());
       return 0;
    }
    const S so = spinor_io<PREC>(out);
    if (mode == PLQCD_HOP_COMPAT) {
-      launch_block<PLQCD_HOP_COMPAT, false, 0, false>(mblock, variant, grid, block, st, si, so, gg, a);
+      lrc = launch_block<PLQCD_HOP_COMPAT, false, 0, false>(mblock, variant, grid, block, st, si, so, gg, a);
    } else if (!dag) {
-      if (epi == 0) launch_block<PLQCD_HOP_WILSON, false, 0, false>(mblock, variant, grid, block, st, si, so, gg, a);
-      else if (!norm) launch_block<PLQCD_HOP_WILSON, false, 1, false>(mblock, variant, grid, block, st, si, so, gg, a);
-      else launch_block<PLQCD_HOP_WILSON, false, 1, true>(mblock, variant, grid, block, st, si, so, gg, a);
+      if (epi == 0) lrc = launch_block<PLQCD_HOP_WILSON, false, 0, false>(mblock, variant, grid, block, st, si, so, gg, a);
+      else if (!norm) lrc = launch_block<PLQCD_HOP_WILSON, false, 1, false>(mblock, variant, grid, block, st, si, so, gg, a);
+      else lrc = launch_block<PLQCD_HOP_WILSON, false, 1, true>(mblock, variant, grid, block, st, si, so, gg, a);
    } else {
-      if (epi == 0) launch_block<PLQCD_HOP_WILSON, true, 0, false>(mblock, variant, grid, block, st, si, so, gg, a);
-      else if (epi == 2) launch_block<PLQCD_HOP_WILSON, true, 2, true>(mblock, variant, grid, block, st, si, so, gg, a, fu);
-      else if (!norm) launch_block<PLQCD_HOP_WILSON, true, 1, false>(mblock, variant, grid, block, st, si, so, gg, a);
-      else launch_block<PLQCD_HOP_WILSON, true, 1, true>(mblock, variant, grid, block, st, si, so, gg, a);
+      if (epi == 0) lrc = launch_block<PLQCD_HOP_WILSON, true, 0, false>(mblock, variant, grid, block, st, si, so, gg, a);
+      else if (epi == 2)
+         lrc = launch_block<PLQCD_HOP_WILSON, true, 2, true>(mblock, variant, grid, block, st, si, so, gg, a, fu);
+      else if (!norm) lrc = launch_block<PLQCD_HOP_WILSON, true, 1, false>(mblock, variant, grid, block, st, si, so, gg, a);
+      else lrc = launch_block<PLQCD_HOP_WILSON, true, 1, true>(mblock, variant, grid, block, st, si, so, gg, a);
    }
+   if (lrc) return lrc;
    PLQCD_HIP_TRY(hipGetLastError());
    return 0;
 }

@@ -185,6 +185,8 @@ int slots_for(plqcd_hip_server *s, const plqcd_hip_lattice *lat)
    const size_t per = plqcd::CgEngine::bytes_needed(lat, s->cfg.sloppy_prec, 1) + (size_t)lat->V * sizeof(spinor);
    size_t fit = (s->budget / 3) / std::max<size_t>(per, 1);
    int S = (int)std::min<size_t>(fit, (size_t)s->cfg.slots);
+   // an engine's fields hold S right-hand sides within the kernels' 32-bit offsets
+   S = std::min(S, plqcd::max_rhs_per_field(lat->Vh));
    return std::max(S, 1);
 }
 
@@ -670,8 +672,12 @@ extern "C" int plqcd_hip_server_create(const plqcd_serve_config *cfg, plqcd_hip_
 extern "C" int plqcd_hip_server_submit(plqcd_hip_server *s, const plqcd_serve_request *req, uint64_t *id)
 {
    if (!s || !req || !id) return plqcd::set_error(1, "server: null argument");
-   for (int i = 0; i < 4; i++)
+   long long V = 1;
+   for (int i = 0; i < 4; i++) {
       if (req->L[i] < 2 || (req->L[i] & 1)) return plqcd::set_error(2, "server: bad lattice extent %d", req->L[i]);
+      V *= req->L[i];
+   }
+   if (V > (1LL << 29)) return plqcd::set_error(2, "server: volume %lld exceeds 2^29 sites", V);
    if (req->nrhs < 1 || !req->b || !req->x || req->max_iter < 1 || !(req->tol > 0))
       return plqcd::set_error(2, "server: bad request (nrhs %d, max_iter %d)", req->nrhs, req->max_iter);
    auto r = std::make_unique<Request>();

@@ -242,7 +242,9 @@ int CgEngine::admit_host(const std::vector<int> &which, const std::vector<const 
 }
 
 // One CG iteration on the sloppy fields, for the slots 0 .. nr-1 (every
-// active slot lies below nr; inactive ones inside get zero steps).
+// active slot lies below nr; inactive ones inside get zero steps).  Device
+// work only: this sequence is what the iteration graphs capture, and no graph
+// holds a copy to or from host memory (launch() reads |r|^2 back after it).
 int CgEngine::iteration(int nr)
 {
    hipStream_t st = lat_->stream;
@@ -268,7 +270,6 @@ int CgEngine::iteration(int nr)
    md.fuse_x = xs_;
    if (int rc = dslash(gs_, qs_, rs_, PLQCD_PARITY_ALL, PLQCD_HOP_WILSON, md)) return rc;
    if (int rc = reduce_partials(partial_, 2 * nc, 0, 2 * nc, nr, rr_new_d(), st)) return rc;
-   PLQCD_HIP_TRY(hipMemcpyAsync(host_, rr_new_d(), S_ * sizeof(double), hipMemcpyDeviceToHost, st));
    PLQCD_HIP_TRY(hipGetLastError());
    return 0;
 }
@@ -327,6 +328,10 @@ int CgEngine::launch()
    } else {
       if (int rc = iteration(nr)) return rc;
    }
+   // |r|^2 of the iteration back to the host, on the stream after the graph
+   // (a memcpy node into host memory inside the graph is executed by the
+   // runtime at hipGraphLaunch; a host SIGSEGV was observed there once)
+   PLQCD_HIP_TRY(hipMemcpyAsync(host_, rr_new_d(), S_ * sizeof(double), hipMemcpyDeviceToHost, lat_->stream));
    PLQCD_HIP_TRY(hipEventRecord(ev_, lat_->stream));
    return 0;
 }

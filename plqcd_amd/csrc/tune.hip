@@ -18,7 +18,9 @@ extern "C" int plqcd_hip_autotune(const plqcd_hip_gauge *g, const plqcd_hip_fiel
    plqcd::clear_error();
    if (!g || !in || !out || reps < 1) return plqcd::set_error(1, "autotune: bad arguments");
    plqcd_hip_lattice *lat = g->lat;
-   const plqcd::TuneKey key{{lat->L[0], lat->L[1], lat->L[2], lat->L[3]}, g->prec, g->links, in->nrhs};
+   // the shapes of plain H applications; the solver's launches keep their own
+   const plqcd::TuneKey key{{lat->L[0], lat->L[1], lat->L[2], lat->L[3]}, g->prec, g->links, in->nrhs,
+                            plqcd::kTuneOpHop};
    struct Cand {
       int order, group;
    };
@@ -92,6 +94,22 @@ extern "C" int plqcd_hip_autotune(const plqcd_hip_gauge *g, const plqcd_hip_fiel
    }
    if (best_ms) *best_ms = best_t;
    return 0;
+}
+
+extern "C" int plqcd_hip_autotune_lookup(const plqcd_hip_gauge *g, const plqcd_hip_field *in, int op, int shape[7])
+{
+   plqcd::clear_error();
+   if (!g || !in || !shape || (op != plqcd::kTuneOpHop && op != plqcd::kTuneOpSolver)) {
+      plqcd::set_error(1, "autotune_lookup: bad arguments");
+      return -1;
+   }
+   const plqcd_hip_lattice *lat = g->lat;
+   const plqcd::TuneKey key{{lat->L[0], lat->L[1], lat->L[2], lat->L[3]}, g->prec, g->links, in->nrhs, op};
+   HopTuning t{};
+   if (!plqcd::lookup_tuning(key, &t)) return 0;
+   const int v[7] = {t.order, t.group, t.block, t.tmarch, t.rb, t.ty, t.tz};
+   for (int k = 0; k < 7; k++) shape[k] = v[k];
+   return 1;
 }
 
 extern "C" int plqcd_hip_autotune_clear(void)

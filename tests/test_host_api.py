@@ -148,3 +148,27 @@ int main(void) {
                     f"-L{lib_dir}", "-lplqcd_hip", f"-Wl,-rpath,{lib_dir}"], check=True)
     r = subprocess.run([str(exe)], capture_output=True, timeout=60)
     assert r.returncode == 0, r
+
+
+def test_offset_limits_rejected_before_any_device_call(hp):
+    """The kernels address sites with 32-bit byte offsets (V/2 * nrhs * 16 <=
+    4 GiB per component block); lattices beyond 2^29 sites are rejected before
+    any device call, and plqcd_hip_max_rhs states the RHS limit of a field."""
+    from plqcd_amd import capi
+    lib = capi.lib()
+    L4 = capi.i4
+    out = C.c_void_p()
+    assert lib.plqcd_hip_lattice_create(L4(1024, 1024, 32, 32), C.byref(out)) == 2  # 2^30 sites
+    assert "2^29" in capi.last_error() and not out.value
+    # 2^32 / (Vh * 16): 48^3 x 96 -> 50, 64^3 x 128 -> 16, 32^3 x 64 -> 256
+    assert lib.plqcd_hip_max_rhs(L4(48, 48, 48, 96)) == 50
+    assert lib.plqcd_hip_max_rhs(L4(64, 64, 64, 128)) == 16
+    assert lib.plqcd_hip_max_rhs(L4(32, 32, 32, 64)) == 256
+    assert lib.plqcd_hip_max_rhs(L4(4, 4, 4, 4)) == 4096
+    assert lib.plqcd_hip_max_rhs(L4(1024, 1024, 16, 32)) == 1      # 2^29 sites: one RHS
+    assert lib.plqcd_hip_max_rhs(L4(1024, 1024, 32, 32)) == 0      # too large
+    assert lib.plqcd_hip_max_rhs(L4(5, 4, 4, 4)) == 0
+    for L in ((48, 48, 48, 96), (64, 64, 64, 128), (32, 32, 32, 64)):
+        V = int(np.prod(L))
+        R = lib.plqcd_hip_max_rhs(L4(*L))
+        assert (V // 2) * R * 16 <= 2 ** 32 < (V // 2) * (R + 1) * 16
