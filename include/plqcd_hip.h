@@ -218,6 +218,16 @@ int plqcd_hip_hop(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip
 int plqcd_hip_apply_m(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip_field *out,
                       const double *kappa_rhs, int dagger);
 
+/* plqcd_hip_apply_m without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops).  No
+ * counterpart in the reference.  The per-RHS kappa goes into a static device
+ * array of 64 doubles per GPU (filled by a kernel from launch arguments; no
+ * host copy, no memset), so in->nrhs may be at most 64 (error 2 above).
+ * Asynchronous on the lattice stream (plqcd_hip_apply_m synchronizes).
+ * Calls that overlap on one GPU share that array and race. */
+int plqcd_hip_apply_m_noalloc(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip_field *out,
+                              const double *kappa_rhs, int dagger);
+
 /* Even-odd (Schur) preconditioned operator on one parity p (EVEN or ODD):
  * out_p = in_p - kappa_r^2 H_{p,q} H_{q,p} in_p  (q the other parity), built
  * from the eo/oe halves of dirac.h:21-22 in wilson mode.  tmp receives

@@ -91,6 +91,7 @@ _PROTOS = {
     "plqcd_hip_field_bytes": (C.c_size_t, [vp]),
     "plqcd_hip_hop": (C.c_int, [vp, vp, vp, C.c_int, C.c_int]),
     "plqcd_hip_apply_m": (C.c_int, [vp, vp, vp, C.POINTER(C.c_double), C.c_int]),
+    "plqcd_hip_apply_m_noalloc": (C.c_int, [vp, vp, vp, C.POINTER(C.c_double), C.c_int]),
     "plqcd_hip_apply_schur": (C.c_int, [vp, vp, vp, vp, C.POINTER(C.c_double), C.c_int]),
     "plqcd_hip_cg_solve": (C.c_int, [vp, vp, vp, vp, C.POINTER(CgParams), C.POINTER(CgStats)]),
     "plqcd_hip_server_create": (C.c_int, [C.POINTER(ServeConfig), C.POINTER(vp)]),
