@@ -12,10 +12,16 @@ boundaries, Eq. 2.1 (wilson mode).  Prints one JSON line.
   cpu_baseline = the reference's CPU hopping matrix (compiled ASSYMBLY eo+oe,
               OpenMP) or, where that binary is absent, the oracle's C/OpenMP
               restatement, timed on this host on a bounded sample
-  formats   = the same H with FP64 12-real links, FP32 18/12-real and FP16
-              12-real storage (200 launches each)
+  formats   = the same H in every storage format: FP64 12-real links, FP32
+              and FP16 with 18/12-real links, FP64 18-real (200 launches each)
+  compat    = the reference's own operator (compat mode), device-resident
+  dropin    = wall time per call of the drop-in eo+oe on host arrays
+              (transfers included)
   solver    = plain FP64 CG on one RHS against the mixed-precision CG on 12
               RHS (FP16 storage, FP64 reliable updates), true residual 1e-10
+  cg_48c96  = the same comparison at 48^3 x 96, one RHS each
+The headline is timed last, after the other GPU measurements, so that its
+warm-up launches start on a GPU at its steady clocks.
 """
 from __future__ import annotations
 
@@ -58,36 +64,76 @@ def read_traffic(kernel_tag: str):
     return float(m.group(1)) if m else None
 
 
+class HopCase:
+    """Resident inputs of one application of H: seeded links and source in
+    HBM (the setup is not timed)."""
+
+    def __init__(self, L, prec, links, mode):
+        from plqcd_amd import hopping as hp
+        self.hp, self.mode = hp, mode
+        V = int(np.prod(L))
+        t0 = time.time()
+        u = hp.random_gauge_su3(V, 1)
+        q = hp.random_spinor(V, 2)
+        log(f"[bench] inputs generated in {time.time() - t0:.1f}s")
+        self.lat = hp.Lattice(L)
+        self.g = hp.Gauge(self.lat, prec, links).upload(u)
+        self.fi = hp.Field(self.lat, prec).upload(q)
+        self.fo = hp.Field(self.lat, prec)
+        self.lat.sync()
+
+    def run(self, steps, warmup):
+        """`warmup` untimed applications, then milliseconds of `steps` applications
+        between two events on the lattice stream (host-synchronised on both sides)."""
+        hp, lat = self.hp, self.lat
+        for _ in range(warmup):
+            hp.hop(self.g, self.fi, self.fo, hp.PARITY_ALL, self.mode)
+        lat.sync()
+        lat.timer_start()
+        for _ in range(steps):
+            hp.hop(self.g, self.fi, self.fo, hp.PARITY_ALL, self.mode)
+        ms = lat.timer_stop()
+        lat.sync()
+        return ms
+
+    def close(self):
+        self.fo.close(); self.fi.close(); self.g.close(); self.lat.close()
+
+
 def gpu_run(L, prec, links, steps, warmup, mode):
-    from plqcd_amd import hopping as hp
-    V = int(np.prod(L))
-    t0 = time.time()
-    u = hp.random_gauge_su3(V, 1)
-    q = hp.random_spinor(V, 2)
-    log(f"[bench] inputs generated in {time.time() - t0:.1f}s")
-    lat = hp.Lattice(L)
-    g = hp.Gauge(lat, prec, links).upload(u)
-    fi = hp.Field(lat, prec).upload(q)
-    fo = hp.Field(lat, prec)
-    del u, q
-    lat.sync()
-    for _ in range(warmup):
-        hp.hop(g, fi, fo, hp.PARITY_ALL, mode)
-    lat.sync()
-    lat.timer_start()
-    for _ in range(steps):
-        hp.hop(g, fi, fo, hp.PARITY_ALL, mode)
-    ms = lat.timer_stop()
-    lat.sync()
-    fo.close(); fi.close(); g.close(); lat.close()
-    return ms
+    c = HopCase(L, prec, links, mode)
+    try:
+        return c.run(steps, warmup)
+    finally:
+        c.close()
+
+
+def host_cores():
+    """(threads to use, description): every CPU this process may run on, which
+    on the benchmark hosts is every physical core (lscpu: sockets x cores per
+    socket, one thread per core)."""
+    avail = len(os.sched_getaffinity(0))
+    phys = None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        f = {k.strip(): v.strip() for k, v in (ln.split(":", 1) for ln in out.splitlines() if ":" in ln)}
+        phys = int(f["Socket(s)"]) * int(f["Core(s) per socket"])
+        tpc = int(f.get("Thread(s) per core", "1"))
+    except Exception:  # lscpu missing: the affinity mask alone
+        tpc = 1
+    cores = min(avail, phys) if phys else avail
+    desc = f"{cores} threads on {avail} CPUs available" + (
+        f", {phys} physical cores ({tpc} thread(s) per core, lscpu)" if phys else "")
+    return cores, desc
 
 
 def cpu_baseline(L, seconds: float):
-    """The reference's own CPU hopping matrix where its binary was built, else the oracle."""
+    """The reference's own CPU hopping matrix where its binary was built, else
+    the oracle, on every host core (OMP_NUM_THREADS = cores, as the reference's
+    NTHREAD, lib/start.c:110-114)."""
     import oracle
     V = int(np.prod(L))
-    cores = min(16, os.cpu_count() or 1)
+    cores, cores_desc = host_cores()
     if oracle.ref_available():
         try:
             with tempfile.TemporaryDirectory() as td:
@@ -101,7 +147,7 @@ def cpu_baseline(L, seconds: float):
                     calls, sec = int(m.group(1)), float(m.group(2))
                     return {"value": round(calls * V * FLOP_PER_SITE / sec / 1e9, 3), "unit": "GFLOP/s",
                             "cores": cores, "kind": "reference",
-                            "sample": f"reference ASSYMBLY eo+oe (gcc -O3 -fopenmp, {cores} threads), "
+                            "sample": f"reference ASSYMBLY eo+oe (gcc -O3 -fopenmp, {cores_desc}), "
                                       f"32^3x64 FP64, {calls} applications of H in {sec:.1f}s, "
                                       f"harness rand() fill"}
                 log(f"[bench] reference CPU binary failed (rc {r.returncode}); using the restatement")
@@ -114,16 +160,17 @@ def cpu_baseline(L, seconds: float):
     n = max(1, int(seconds / max(t1, 1e-3)))
     t = oracle.bench(q, u, L, n, oracle.MODE_WILSON, cores)
     return {"value": round(n * V * FLOP_PER_SITE / t / 1e9, 3), "unit": "GFLOP/s", "cores": cores, "kind": "port",
-            "sample": f"C/OpenMP restatement eo+oe ({cores} threads), 32^3x64 FP64, {n} applications of H in {t:.1f}s"}
+            "sample": f"C/OpenMP restatement eo+oe ({cores_desc}), 32^3x64 FP64, {n} applications of H in {t:.1f}s"}
 
 
 def formats_run(L, mode, steps=200, warmup=10):
-    """The same application of H in the other storage formats (FP64 with 12-real
-    links, FP32 and FP16 with 18- and 12-real links), default launch shapes;
-    per format: time per launch, GFLOP/s and compulsory GB/s."""
+    """The same application of H in every storage format (FP64 with 12-real
+    links, FP32 and FP16 with 18- and 12-real links, and last FP64 with
+    18-real links, the headline's own format), default launch shapes; per
+    format: time per launch, GFLOP/s and compulsory GB/s."""
     V = int(np.prod(L))
     out = {}
-    for prec, links in ((0, 12), (1, 18), (1, 12), (2, 18), (2, 12)):
+    for prec, links in ((0, 12), (1, 18), (1, 12), (2, 18), (2, 12), (0, 18)):
         ms = gpu_run(L, prec, links, steps, warmup, mode)
         t = ms / 1e3 / steps
         gbs = BYTES_PER_SITE[(prec, links)] * V / t / 1e9
@@ -132,6 +179,66 @@ def formats_run(L, mode, steps=200, warmup=10):
                                                "compulsory_bytes_per_site": BYTES_PER_SITE[(prec, links)],
                                                "compulsory_GBps": round(gbs, 1),
                                                "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 4)}
+    return out
+
+
+def compat_run(L, steps=200, warmup=10):
+    """The reference's own operator (compat mode: lib/dirac.c as compiled,
+    tests/test_hopping.c:287-301), FP64 18-real, device-resident fields."""
+    V = int(np.prod(L))
+    ms = gpu_run(L, 0, 18, steps, warmup, 0)
+    t = ms / 1e3 / steps
+    return {"us_per_launch": round(t * 1e6, 2), "gflops": round(FLOP_PER_SITE * V / t / 1e9, 1),
+            "frac_of_hbm_peak": round(960 * V / t / 1e9 / HBM_PEAK_GBS, 4)}
+
+
+def dropin_run(L, calls=5):
+    """Wall seconds per call of the drop-in plqcd_hopping_matrix_hip (eo then
+    oe on host arrays in the reference layout, compat mode): the upload of
+    qin and the links, both halves of H and the download of qout
+    (tests/test_hopping.c:287-301 times the same eo+oe pair on host arrays)."""
+    from plqcd_amd import hopping as hp
+    V = int(np.prod(L))
+    u = hp.random_gauge_su3(V, 1)
+    q = hp.random_spinor(V, 2)
+    out = hp.spinor_field(V)
+    hp.init_dims(L)
+    hp.set_mode(hp.HOP_COMPAT)
+    hp.hopping_matrix(q, out, u)  # device buffers and first-touch costs
+    ts = [hp.hopping_matrix(q, out, u) for _ in range(calls)]
+    hp.finalize_plqcd()
+    t = float(np.median(ts))
+    host_bytes = V * (192 + 576 + 192)
+    return {"calls": calls, "seconds_per_call": round(t, 4), "gflops": round(FLOP_PER_SITE * V / t / 1e9, 2),
+            "host_bytes_per_call": host_bytes, "host_GBps": round(host_bytes / t / 1e9, 2)}
+
+
+def cg48_compare(kappa=0.124, tol=1e-10):
+    """48^3 x 96 CG to a 1e-10 true residual: plain FP64 against the mixed
+    FP16/12-real inner solve with FP64 reliable updates (one RHS each; the
+    second of two solves timed)."""
+    from plqcd_amd import hopping as hp, solver as sv
+    L = (48, 48, 48, 96)
+    V = int(np.prod(L))
+    u = hp.random_gauge_su3(V, 1)
+    lat = hp.Lattice(L)
+    g = hp.Gauge(lat).upload(u)
+    gs = hp.Gauge(lat, 2, 12).upload(u)
+    del u
+    b = hp.Field(lat).upload(hp.random_spinor(V, 2))
+    x = hp.Field(lat)
+    out = {"lattice": list(L), "kappa": kappa, "tol": tol}
+    for label, sl in (("fp64", None), ("mixed_f16_l12", gs)):
+        sv.cg_solve(g, b, x, kappa, tol, sloppy=sl)
+        lat.sync()
+        t0 = time.perf_counter()
+        st = sv.cg_solve(g, b, x, kappa, tol, sloppy=sl)[0]
+        t = time.perf_counter() - t0
+        out[label] = {"seconds": round(t, 4), "iterations": st["iterations"],
+                      "reliable_updates": st["reliable_updates"], "true_residual": st["true_residual"],
+                      "converged": bool(st["converged"])}
+    out["speedup"] = round(out["fp64"]["seconds"] / out["mixed_f16_l12"]["seconds"], 3)
+    x.close(); b.close(); gs.close(); g.close(); lat.close()
     return out
 
 
@@ -479,13 +586,32 @@ def main():
     links = args.links or (18 if prec < 2 else 12)
     mode = 1 if args.mode == "wilson" else 0
 
-    ms = gpu_run(L, prec, links, args.steps, args.warmup, mode)
+    headline = prec == 0 and links == 18 and mode == 1 and L == (32, 32, 32, 64)
+    # The CPU baseline first (the GPU idles meanwhile), then the GPU figures
+    # with the headline's inputs already resident, and the headline last, right
+    # after the storage formats (the last of which is this same kernel, 200
+    # launches): after about 2 s without work the GPU needs some 20 ms of load
+    # to return to its steady clocks (20 applications after an idle gap take
+    # 448 us each, 413 us once warm), which a short timed region would
+    # otherwise measure instead of the kernel.
+    cpu = None if args.no_cpu else cpu_baseline(L, args.cpu_seconds)
+    case = HopCase(L, prec, links, mode)
+    compat = compat_run(L) if headline and not args.no_formats else None
+    dropin = dropin_run(L) if headline and not args.no_formats else None
+    solver = None if args.no_solver else solve_compare(L)
+    cg48 = cg48_compare() if headline and not args.no_solver else None
+    formats = None if (args.no_formats or prec != 0) else formats_run(L, mode)
+    ms = case.run(args.steps, args.warmup)
+    case.close()
     t_launch = ms / 1e3 / args.steps
     gflops = FLOP_PER_SITE * V / t_launch / 1e9
     bytes_launch = BYTES_PER_SITE[(prec, links)] * V
     achieved = bytes_launch / t_launch / 1e9
     tag = f"hop_{PREC_NAMES[prec]}_l{links}_{'x'.join(map(str, L))}"
-    cpu = None if args.no_cpu else cpu_baseline(L, args.cpu_seconds)
+    traffic = read_traffic(tag)
+    traffic_src = (f"profiles/{tag}_pmc.txt: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of this kernel at this "
+                   f"shape, 2 x FETCH + WRITE (copy-kernel calibration), committed, not measured in this run"
+                   if traffic else None)
     rec = {
         "metric": METRIC,
         "value": round(gflops, 2),
@@ -504,10 +630,14 @@ def main():
                    "lattice": list(L), "sites": V, "flop_per_site": FLOP_PER_SITE,
                    "bytes_per_site": BYTES_PER_SITE[(prec, links)], "parallelism": "1 GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(tag)},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_source": traffic_src},
         "cpu_baseline": cpu,
-        "formats": None if (args.no_formats or prec != 0) else formats_run(L, mode),
-        "solver": None if args.no_solver else solve_compare(L),
+        "formats": formats,
+        "compat": compat,
+        "dropin": dropin,
+        "solver": solver,
+        "cg_48c96": cg48,
     }
     print(json.dumps(rec), flush=True)
 

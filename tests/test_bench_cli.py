@@ -33,3 +33,15 @@ def test_headline_traffic_comes_from_the_profile():
     compulsory = bench.BYTES_PER_SITE[(0, 18)] * 32 * 32 * 32 * 64
     assert t is not None and 1.0 <= t / compulsory < 2.0
     assert bench.read_traffic("no_such_kernel") is None
+
+
+def test_cpu_baseline_uses_every_available_core():
+    """cpu_baseline runs on every CPU this process may use (the physical cores
+    of the benchmark hosts), and says how many."""
+    import os
+    sys.path.insert(0, str(ROOT))
+    import bench
+    cores, desc = bench.host_cores()
+    avail = len(os.sched_getaffinity(0))
+    assert 1 <= cores <= avail
+    assert f"{avail} CPUs available" in desc

@@ -312,7 +312,8 @@ TILES = [((8, 8, 8, 16), [(4, 4), (8, 2), (2, 8), (8, 4), (4, 8)]),
 
 
 @pytest.mark.parametrize("L,tiles", TILES)
-@pytest.mark.parametrize("prec,links,tol", [(0, 18, TOL64), (1, 12, TOL32), (2, 12, TOL16), (2, 18, TOL16)])
+@pytest.mark.parametrize("prec,links,tol", [(0, 18, TOL64), (0, 12, TOL64), (1, 12, TOL32), (2, 12, TOL16),
+                                            (2, 18, TOL16)])
 def test_tile_kernel_bitwise_equal_tmarch(hp, orc, L, tiles, prec, links, tol):
     """The LDS-tiled t-march gives bitwise the t-march's and the gather
     kernel's output for every tile (including tiles spanning a whole extent,
