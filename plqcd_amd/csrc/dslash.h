@@ -52,10 +52,10 @@ namespace plqcd {
 //   alpha   != null: CG update (with dag, kappa and partial): Ap = in - kappa H^+ in
 //                    is not stored; fuse_x += alpha[r] fuse_p and out -= alpha[r] Ap
 //                    (out holds the residual), whose |out|^2 goes to partial
-//   resid_b != null: residual (with dag, kappa, partial and mask; F64 storage):
-//                    out = resid_b - (in - kappa H^+ in) for the RHS whose mask[r]
-//                    is set, stored in out's own precision (any), and its |out|^2
-//                    to partial (0 for the other RHS, whose out is left alone)
+//   resid_b != null: residual (with dag, kappa and partial; F64 storage):
+//                    out = resid_b - (in - kappa H^+ in) for every RHS of the
+//                    launch, stored in out's own precision (any), and its |out|^2
+//                    to partial
 //   nr      > 0:     only the right-hand sides r0 .. r0+nr-1 of the fields (kappa,
 //                    alpha and partial stay indexed by the field's RHS)
 struct DslashOp {
@@ -67,7 +67,6 @@ struct DslashOp {
    const plqcd_hip_field *fuse_p = nullptr;
    plqcd_hip_field *fuse_x = nullptr;
    const plqcd_hip_field *resid_b = nullptr;
-   const int *mask = nullptr;
 };
 
 int dslash(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip_field *out, int parity, int mode,

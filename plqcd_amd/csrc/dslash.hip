@@ -154,9 +154,9 @@ int plqcd::dslash(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip
    if (op.resid_b) {
       const plqcd_hip_field *b = op.resid_b;
       if (g->prec != PLQCD_PREC_F64 || b->lat != g->lat || b->prec != PLQCD_PREC_F64 || b->nrhs != in->nrhs ||
-          b == in || b == out || op.alpha || !op.dag || !op.kappa || !op.partial || !op.mask)
+          b == in || b == out || op.alpha || !op.dag || !op.kappa || !op.partial)
          return plqcd::set_error(1, "hop: the residual epilogue needs an F64 gauge, an F64 b distinct from in and "
-                                    "out, M^+, kappa, a norm and a mask");
+                                    "out, M^+, kappa and a norm");
    }
    switch (g->prec * 100 + g->links) {
       case PLQCD_PREC_F64 * 100 + 18: return plqcd::hop_launch<PLQCD_PREC_F64, 18>(g, in, out, parity, mode, op);

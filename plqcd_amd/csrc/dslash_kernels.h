@@ -418,7 +418,6 @@ template <class SIO>
 struct CgFuse {
    SIO p, x;  // EPI 3: p holds b
    const double *alpha;
-   const int *mask;  // EPI 3: RHS whose residual is formed
 };
 
 // Epilogue: EPI = 0 stores H in; EPI = 1 stores in(x) - kappa_r * (H in)(x).
@@ -427,9 +426,12 @@ struct CgFuse {
 // Ap (out holds the residual), in the working precision of the kernel (float
 // for F32/F16 storage, double for F64).
 // EPI = 3 (F64 storage) stores the residual b - Ap, with b in f.p, in the
-// precision of `out` for the RHS whose f.mask entry is set (the reliable
-// update of the mixed-precision CG: the true residual goes straight into the
-// low-precision iterate's residual).
+// precision of `out`, for every RHS of the launch (the reliable update of the
+// mixed-precision CG: the true residual goes straight into the low-precision
+// iterate's residual; the solver launches it once per run of consecutive
+// RHS being updated).  No per-RHS predicate: a data-dependent condition
+// around the stores made the compiler keep both paths' values live (280
+// registers, one wave per SIMD, against 144 and three waves).
 // With NORM the site's |out|^2 joins the deterministic chunk reduction.  `in`
 // and `out` are positioned at the RHS `r`, whose kappa and alpha apply.
 template <int EPI, bool NORM, class SIO, class SO>
@@ -446,8 +448,6 @@ __device__ __forceinline__ double epilogue(const SIO &in, const SO &out, const H
       for (int m = 0; m < 12; m++) acc[m] = cxmky(cen ? cen[m] : c0[m], kap, acc[m]);
    }
    if constexpr (EPI == 3) {
-      if (!f.mask[r]) return 0.0;
-      __builtin_amdgcn_sched_barrier(0);  // b is read after H: no early loads holding registers
       const SIO b = f.p.at_rhs(r);
       const uint32_t o = b.off(j);
       d2 rs[12];
@@ -1004,7 +1004,7 @@ int launch_one(int variant, dim3 grid, dim3 block, hipStream_t st, S si, SO so, 
 {
    constexpr bool same = std::is_same<S, SO>::value && EPI != 3;  // the tile and RHS-blocked kernels
    if (variant == V_MARCH) {
-      if constexpr (MODE == PLQCD_HOP_WILSON && EPI != 3) {
+      if constexpr (MODE == PLQCD_HOP_WILSON) {
          hipLaunchKernelGGL((hop_tmarch_kernel<DAG, EPI, NORM, BLOCK, S, G, SO>), grid, block, 0, st, si, so, gg, a, f);
          return 0;
       }
@@ -1087,8 +1087,8 @@ int launch_t(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip_fiel
       return plqcd::set_error(2, "hop: M, M^+ and norms need the wilson mode");
    if (epi == 2 && (!dag || !norm || !op.kappa || !op.fuse_p || !op.fuse_x))
       return plqcd::set_error(2, "hop: the CG-update epilogue needs M^+, a norm, kappa, p and x");
-   if (epi == 3 && (PREC != PLQCD_PREC_F64 || !dag || !norm || !op.kappa || !op.mask))
-      return plqcd::set_error(2, "hop: the residual epilogue needs F64 storage, M^+, a norm, kappa and a mask");
+   if (epi == 3 && (PREC != PLQCD_PREC_F64 || !dag || !norm || !op.kappa))
+      return plqcd::set_error(2, "hop: the residual epilogue needs F64 storage, M^+, a norm and kappa");
    CgFuse<S> fu{};
    if (epi == 2) {
       fu.p = spinor_io<PREC>(op.fuse_p);
@@ -1096,18 +1096,14 @@ int launch_t(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip_fiel
       fu.alpha = op.alpha;
    } else if (epi == 3) {
       fu.p = spinor_io<PREC>(op.resid_b);
-      fu.mask = op.mask;
    }
    if (norm && a.nparity != 2) return plqcd::set_error(2, "hop: norms are only fused into full-lattice launches");
    const int T = tu.tmarch;
    // the t-march kernel serves full-lattice wilson applications whose time
    // extent is a multiple of T (and, with a norm, whose t-slices hold whole
    // reduction chunks)
-   // (not with the residual epilogue: at 32^3 x 64 and 12 RHS that t-march
-   // holds 364 registers, 108 of them AGPRs, and took 13 ms, the gather
-   // kernel 7.2 ms)
-   const bool march = !resid && rb == 1 && T > 0 && mode == PLQCD_HOP_WILSON && a.nparity == 2 &&
-                      lat->L[3] % T == 0 && (!norm || a.S % REDUCE_CHUNK == 0);
+   const bool march = rb == 1 && T > 0 && mode == PLQCD_HOP_WILSON && a.nparity == 2 && lat->L[3] % T == 0 &&
+                      (!norm || a.S % REDUCE_CHUNK == 0);
    // the LDS-tiled t-march: one RHS, a tile of whole x rows dividing the y and
    // z extents, 2 Lxh ty tz lanes per block (at most 256 with F64 storage for
    // the LDS), and, with a norm, reduction chunks that do not leave a tile row

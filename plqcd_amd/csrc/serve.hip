@@ -31,6 +31,7 @@
 #include <chrono>
 #include <condition_variable>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <set>
 #include <string>
@@ -656,6 +657,19 @@ extern "C" int plqcd_hip_server_create(const plqcd_serve_config *cfg, plqcd_hip_
    if (!s) return plqcd::set_error(4, "out of host memory");
    s->cfg = *cfg;
    PLQCD_HIP_TRY(hipSetDevice(cfg->device));
+   // one stream per engine and per transfer thread: with HIP's default of 4
+   // hardware queues per process unrelated streams share queues and wait for
+   // each other (a 60 s request stream took 53.6 s of GPU work instead of
+   // 36.4 s); the variable is read when HIP initialises, so only warn here
+   {
+      const char *q = std::getenv("GPU_MAX_HW_QUEUES");
+      const int nq = q ? std::atoi(q) : 4;
+      if (nq < 8)
+         std::fprintf(stderr,
+                      "plqcd server: GPU_MAX_HW_QUEUES is %d; set it to 16 in the environment before the process "
+                      "starts using the GPU, or engine and transfer streams share hardware queues\n",
+                      nq);
+   }
    size_t free_b = 0, total_b = 0;
    PLQCD_HIP_TRY(hipMemGetInfo(&free_b, &total_b));
    const size_t margin = (size_t)8 << 30;  // the runtime's own allocations, transfer stagers, fragmentation

@@ -336,8 +336,9 @@ int CgEngine::launch()
    return 0;
 }
 
-// x += xs ; r = b - M^+M x (into t2) ; rs = r ; tmp <- |r|^2, for the slots in
-// update; the FP64 operator runs only over the window spanning them.
+// x += xs ; r = b - M^+M x ; rs = r ; tmp <- |r|^2, for the slots in update.
+// The FP64 operators run once per run of consecutive slots being updated
+// (their epilogue forms the residual of every RHS it covers, see EPI 3).
 int CgEngine::reliable_update()
 {
    hipStream_t st = lat_->stream;
@@ -345,19 +346,27 @@ int CgEngine::reliable_update()
    const Window w = window_of(update_);
    if (w.nr == 0) return 0;
    if (int rc = blas_accum(x_, xs_, 1, update_d(), w)) return rc;
-   DslashOp m;
-   m.r0 = w.r0;
-   m.nr = w.nr;
-   m.kappa = kappa_d();
-   if (int rc = dslash(g_, x_, t1_, PLQCD_PARITY_ALL, PLQCD_HOP_WILSON, m)) return rc;
-   // r = b - M^+ t1 in FP64, stored straight into the iterate's residual in
-   // its own precision, with |r|^2, for the RHS being updated
-   m.dag = 1;
-   m.partial = partial_;
-   m.resid_b = b_;
-   m.mask = update_d();
-   if (int rc = dslash(g_, t1_, rs_, PLQCD_PARITY_ALL, PLQCD_HOP_WILSON, m)) return rc;
-   if (int rc = reduce_partials(partial_, 2 * nc, 0, 2 * nc, w.nr, tmp_d(), st, w.r0)) return rc;
+   for (int s = w.r0; s < w.r0 + w.nr;) {
+      if (!update_[s]) {
+         s++;
+         continue;
+      }
+      int e = s;
+      while (e < w.r0 + w.nr && update_[e]) e++;
+      DslashOp m;
+      m.r0 = s;
+      m.nr = e - s;
+      m.kappa = kappa_d();
+      if (int rc = dslash(g_, x_, t1_, PLQCD_PARITY_ALL, PLQCD_HOP_WILSON, m)) return rc;
+      // r = b - M^+ t1 in FP64, stored straight into the iterate's residual in
+      // its own precision, with |r|^2
+      m.dag = 1;
+      m.partial = partial_;
+      m.resid_b = b_;
+      if (int rc = dslash(g_, t1_, rs_, PLQCD_PARITY_ALL, PLQCD_HOP_WILSON, m)) return rc;
+      if (int rc = reduce_partials(partial_, 2 * nc, 0, 2 * nc, e - s, tmp_d(), st, s)) return rc;
+      s = e;
+   }
    PLQCD_HIP_TRY(hipMemcpyAsync(host_ + S_, tmp_d(), S_ * sizeof(double), hipMemcpyDeviceToHost, st));
    return 0;
 }
