@@ -182,6 +182,19 @@ def formats_run(L, mode, steps=200, warmup=10):
     return out
 
 
+def mix_ceiling(L, reps=20):
+    """GB/s of a streaming kernel with the FP64 hopping matrix's byte counts
+    and read:write mix at this lattice: four streams of V x 192 B read, one of
+    V x 192 B written (plqcd_hip_stream_bandwidth kind 3); the practical
+    memory ceiling for H's traffic, measured in the same run."""
+    import ctypes as C
+    from plqcd_amd import capi
+    V = int(np.prod(L))
+    gbs = C.c_double()
+    capi.check(capi.lib().plqcd_hip_stream_bandwidth(3, V * 192, reps, C.byref(gbs)), "stream_bandwidth")
+    return round(gbs.value, 1)
+
+
 def compat_run(L, steps=200, warmup=10):
     """The reference's own operator (compat mode: lib/dirac.c as compiled,
     tests/test_hopping.c:287-301), FP64 18-real, device-resident fields."""
@@ -412,11 +425,16 @@ def calib_suite():
     import ctypes as C
     from plqcd_amd import capi, hopping as hp
     lib = capi.lib()
-    for kind, name in ((0, "copy"), (1, "read"), (2, "write")):
+    for kind, name in ((0, "copy"), (1, "read"), (2, "write"), (3, "mix41")):
         gbs = C.c_double()
         capi.check(lib.plqcd_hip_stream_bandwidth(kind, 1 << 30, 10, C.byref(gbs)), "stream_bandwidth")
         emit({"suite": "calib", "case": f"stream_{name}_1GiB", "bytes_per_buffer": 1 << 30, "timed_launches": 10,
               "GBps": round(gbs.value, 1)})
+    # the FP64 hopping matrix's own byte counts at 32^3 x 64: 4 x 402 MB read, 402 MB written
+    gbs = C.c_double()
+    capi.check(lib.plqcd_hip_stream_bandwidth(3, 2097152 * 192, 20, C.byref(gbs)), "stream_bandwidth")
+    emit({"suite": "calib", "case": "stream_mix41_hop_f64_32x32x32x64", "bytes_per_buffer": 2097152 * 192,
+          "timed_launches": 20, "GBps": round(gbs.value, 1)})
     # the same probes over working sets that fit the 256 MB Infinity Cache
     # (read 128 MiB, copy 2 x 128 MiB) and that do not (4 GiB)
     for kind, name, mib, reps in ((1, "read", 128, 100), (0, "copy", 128, 100), (1, "read", 4096, 10),
@@ -600,6 +618,7 @@ def main():
     dropin = dropin_run(L) if headline and not args.no_formats else None
     solver = None if args.no_solver else solve_compare(L)
     cg48 = cg48_compare() if headline and not args.no_solver else None
+    ceiling = mix_ceiling(L) if headline else None
     formats = None if (args.no_formats or prec != 0) else formats_run(L, mode)
     ms = case.run(args.steps, args.warmup)
     case.close()
@@ -631,7 +650,10 @@ def main():
                    "bytes_per_site": BYTES_PER_SITE[(prec, links)], "parallelism": "1 GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "traffic_source": traffic_src},
+                     "traffic_source": traffic_src,
+                     "measured_ceiling": None if ceiling is None else {
+                         "probe": "stream kernel with H's byte counts: 4 x V x 192 B read, V x 192 B written",
+                         "GBps": ceiling, "frac_of_ceiling": round(achieved / ceiling, 4)}},
         "cpu_baseline": cpu,
         "formats": formats,
         "compat": compat,

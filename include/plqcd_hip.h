@@ -374,8 +374,10 @@ int plqcd_hip_hop_batch(int n, const plqcd_hip_gauge *const *g, const plqcd_hip_
                         plqcd_hip_field *const *out, int mode);
 
 /* HBM probe: achieved GB/s of a streaming kernel over `bytes` per buffer,
- * best of `reps` back-to-back launches timed together.  kind 0: copy (reads
- * and writes `bytes`); 1: read only; 2: write only. */
+ * `reps` back-to-back launches timed together.  kind 0: copy (reads and
+ * writes `bytes`); 1: read only; 2: write only; 3: the hopping matrix's
+ * read:write mix of 4:1 (reads 4 x `bytes`, writes `bytes`; the best of a
+ * sweep of grid sizes). */
 int plqcd_hip_stream_bandwidth(int kind, size_t bytes, int reps, double *gbs);
 
 /* ------------------------------------------------ host memory */
