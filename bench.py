@@ -418,10 +418,12 @@ def solver_suite():
 
 
 def calib_suite():
-    """HBM probes of known byte counts (copy 1 GiB, read 1 GiB, write 1 GiB;
-    one untimed and 10 timed launches each), then two applications of H per
-    storage format at 32^3 x 64 with the default launch shapes: one run holds
-    the counter calibration and the kernels."""
+    """HBM probes of known byte counts (copy, read, write and the 4:1 mix over
+    1 GiB, the mix at H's own sizes, Infinity-Cache- and HBM-sized working
+    sets), then two applications of H per storage format at 32^3 x 64 with the
+    default launch shapes, M on 12 RHS (FP64) and H on 12 RHS (FP64 18-real,
+    FP16 12-real, and FP16 under two other block orders): one run holds the
+    counter calibration and the kernels."""
     import ctypes as C
     from plqcd_amd import capi, hopping as hp
     lib = capi.lib()
@@ -469,6 +471,26 @@ def calib_suite():
     emit({"suite": "calib", "case": "apply_m_f64_l18_32x32x32x64_R12", "applications": 2,
           "compulsory_bytes": (12 * (192 + 192) + 576) * V})
     fi.close(); fo.close(); g.close()
+    # 12 RHS: H with the multi-RHS default shapes (FP64 18-real, FP16 12-real),
+    # then the FP16 H under the other block orders (plain, contiguous per XCD)
+    from plqcd_amd import capi as _capi
+    for prec, links, tune, tag in ((0, 18, None, "default"), (2, 12, None, "default"),
+                                   (2, 12, (0, 1, 256, 8, 1, 0, 0), "order0_T8"),
+                                   (2, 12, (1, 1, 256, 8, 1, 0, 0), "order1_T8")):
+        if tune:
+            _capi.check(lib.plqcd_hip_hop_tuning(prec, *tune), "hop_tuning")
+        g = hp.Gauge(lat, prec, links).upload(u)
+        fi, fo = hp.Field(lat, prec, 12), hp.Field(lat, prec, 12)
+        for r in range(12):
+            fi.upload(hp.random_spinor(V, 10 + r), r)
+        for _ in range(2):
+            hp.hop(g, fi, fo, hp.PARITY_ALL, hp.HOP_WILSON)
+        lat.sync()
+        link_b = BYTES_PER_SITE[(prec, links)] - (384 if prec == 0 else 104)
+        emit({"suite": "calib", "case": f"hop_{PREC_NAMES[prec]}_l{links}_32x32x32x64_R12_{tag}", "applications": 2,
+              "compulsory_bytes": (12 * (BYTES_PER_SITE[(prec, links)] - link_b) + link_b) * V})
+        fi.close(); fo.close(); g.close()
+        lib.plqcd_hip_hop_tuning_reset(prec)
     lat.close()
 
 
