@@ -259,11 +259,14 @@ def cg48_compare(kappa=0.124, tol=1e-10):
     return out
 
 
-def solve_compare(L, nrhs=12, sloppy=(2, 12), kappa=0.124, tol=1e-10, reps=3):
+def solve_compare(L, nrhs=12, sloppy=(2, 12), kappa=0.124, tol=1e-10, reps=3, tune=False):
     """Plain FP64 CG on one RHS vs the mixed-precision CG on a batch of nrhs,
     same gauge field and sources, true residual <= tol; each solve timed
-    `reps` times after one untimed solve, the median reported.  Returns a dict."""
-    from plqcd_amd import hopping as hp, solver as sv
+    `reps` times after one untimed solve, the median reported.  With `tune`,
+    both solvers' launch shapes are autotuned first (plqcd_hip_autotune_solver
+    on the iterating gauge field and RHS count of each).  Returns a dict."""
+    import ctypes as C
+    from plqcd_amd import capi, hopping as hp, solver as sv
     V = int(np.prod(L))
     u = hp.random_gauge_su3(V, 1)
     lat = hp.Lattice(L)
@@ -275,6 +278,15 @@ def solve_compare(L, nrhs=12, sloppy=(2, 12), kappa=0.124, tol=1e-10, reps=3):
         x = hp.Field(lat, 0, R)
         for r in range(R):
             b.upload(hp.random_spinor(V, 100 + r), r)
+        shape = None
+        if tune:
+            gi = sl if sl is not None else g
+            fsl = hp.Field(lat, gi.prec, R)
+            best = (C.c_int * 7)()
+            bms = C.c_double()
+            capi.check(capi.lib().plqcd_hip_autotune_solver(gi.h, fsl.h, 5, best, C.byref(bms)), "autotune_solver")
+            fsl.close()
+            shape = {"shape": list(best), "ms_per_iteration_pair": round(bms.value, 4)}
         sv.cg_solve(g, b, x, kappa, tol, sloppy=sl)  # graph capture and first-touch costs
         ts = []
         for _ in range(reps):
@@ -289,11 +301,15 @@ def solve_compare(L, nrhs=12, sloppy=(2, 12), kappa=0.124, tol=1e-10, reps=3):
                       "reliable_updates": max(s_["reliable_updates"] for s_ in st),
                       "max_true_residual": max(s_["true_residual"] for s_ in st),
                       "all_converged": all(s_["converged"] for s_ in st)}
+        if shape is not None:
+            out[label]["tuned"] = shape
         b.close(); x.close()
     keys = list(out)
     out["ratio_solves_per_s"] = round(out[keys[1]]["solves_per_s"] / out[keys[0]]["solves_per_s"], 3)
     out["lattice"] = list(L)
     out["kappa"], out["tol"] = kappa, tol
+    if tune:
+        capi.lib().plqcd_hip_autotune_clear()
     gs.close(); g.close(); lat.close()
     return out
 
@@ -361,6 +377,10 @@ def solver_suite():
             rec = solve_compare(L, nrhs=R, sloppy=sl)
             rec.update({"suite": "solver", "case": f"cg_32c64_mixed_{PREC_NAMES[sl[0]]}_R{R}_vs_fp64_R1"})
             emit(rec)
+    # both solvers with their launch shapes autotuned first (FP16 12-real, 12 RHS)
+    rec = solve_compare(L, nrhs=12, sloppy=(2, 12), tune=True)
+    rec.update({"suite": "solver", "case": "cg_32c64_mixed_f16_R12_vs_fp64_R1_solver_autotuned"})
+    emit(rec)
     # ragged batch: 16^3x32, 24^3x48, 32^3x64, each 12 RHS, one launch
     shapes = [(16, 16, 16, 32), (24, 24, 24, 48), (32, 32, 32, 64)]
     sources = [[hp.random_spinor(int(np.prod(Ls)), 50 + r) for r in range(12)] for Ls in shapes]

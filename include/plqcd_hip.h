@@ -355,10 +355,22 @@ int plqcd_hip_hop_get_tuning(int prec, int out[7]);
  * application.  in/out are overwritten. */
 int plqcd_hip_autotune(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip_field *out, int reps,
                        int best[7], double *best_ms);
+/* The same for the solvers' launches over all RHS of `in` (M = 1 - kappa H
+ * with |Mp|^2, then M^+ fused with the CG update and |r|^2, and the residual
+ * launch of reliable updates): times the pair of launches of one CG iteration
+ * with `reps` timed pairs per shape, on scratch fields it allocates (`in`,
+ * whose storage precision must be the gauge field's, is only read), and
+ * makes the fastest the shape of those launches from now on; *best_ms is its
+ * time per pair.  Plain applications of H keep their own shapes.  Returns 0,
+ * or nonzero on bad arguments or a device error (the previous shape stays).
+ * No counterpart in the reference, which has no solver and chooses among
+ * its hopping variants by hand (tests/test_hopping.c:283-560). */
+int plqcd_hip_autotune_solver(const plqcd_hip_gauge *g, const plqcd_hip_field *in, int reps, int best[7],
+                              double *best_ms);
 /* The launch shape cached for the shape of (g, in) -- lattice, precision,
  * link format, RHS count -- and operation `op`: 0 plain applications of H
- * (what plqcd_hip_autotune tunes), 1 the solvers' M / M^+ launches (which
- * keep their built-in shapes).  Returns 1 and fills shape = {order, group,
+ * (what plqcd_hip_autotune tunes), 1 the solvers' M / M^+ launches (what
+ * plqcd_hip_autotune_solver tunes; built-in shapes until then).  Returns 1 and fills shape = {order, group,
  * block, tmarch, rb, tile_y, tile_z} when one is cached, 0 when none is, -1
  * on bad arguments.  No counterpart in the reference, which chooses among its
  * variants by hand (tests/test_hopping.c:283-560). */

@@ -103,6 +103,7 @@ _PROTOS = {
     "plqcd_hip_autotune": (C.c_int, [vp, vp, vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     "plqcd_hip_autotune_clear": (C.c_int, []),
     "plqcd_hip_autotune_lookup": (C.c_int, [vp, vp, C.c_int, C.POINTER(C.c_int)]),
+    "plqcd_hip_autotune_solver": (C.c_int, [vp, vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     "plqcd_hip_hop_batch": (C.c_int, [C.c_int, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.c_int]),
     "plqcd_hip_hop_tuning": (C.c_int, [C.c_int] * 8),
     "plqcd_hip_hop_get_tuning": (C.c_int, [C.c_int, C.POINTER(C.c_int)]),

@@ -1,26 +1,27 @@
-// Autotuner of the hopping kernels' launch shape.
+// Autotuners of the hopping kernels' launch shape.
 //
-// For one (lattice, storage precision, link format, RHS count) it times every
+// For one (lattice, storage precision, link format, RHS count) they time every
 // candidate launch shape -- block->tile order (plain, contiguous per XCD, XCD
 // runs of 8 or 16 tiles), threads per block, t-march length (gather kernel or
 // T t-slices per thread, T dividing the time extent), right-hand sides per
 // lane (RHS-blocked gather) and, for one RHS, the LDS tile of the t-march
 // (y and z extents whose whole-x-row tiles give 128, 256 or 512 lanes) --
-// with HIP events on the lattice stream, keeps the fastest in the per-shape
-// cache that every later hopping launch of that shape consults, and returns it.
+// with HIP events on the lattice stream, keep the fastest in the per-shape
+// cache that every later launch of that shape consults, and return it.
+// plqcd_hip_autotune times plain applications of H; plqcd_hip_autotune_solver
+// times the pair of launches of one CG iteration (M p with |q|^2, then M^+ q
+// fused with the x and r updates and |r|^2), whose shapes are cached apart.
 #include <vector>
 
+#include "blas.h"
 #include "dslash.h"
+#include "reduce.h"
 
-extern "C" int plqcd_hip_autotune(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip_field *out,
-                                  int reps, int best[7], double *best_ms)
+namespace {
+
+std::vector<HopTuning> candidates(const plqcd_hip_gauge *g, const plqcd_hip_field *in)
 {
-   plqcd::clear_error();
-   if (!g || !in || !out || reps < 1) return plqcd::set_error(1, "autotune: bad arguments");
-   plqcd_hip_lattice *lat = g->lat;
-   // the shapes of plain H applications; the solver's launches keep their own
-   const plqcd::TuneKey key{{lat->L[0], lat->L[1], lat->L[2], lat->L[3]}, g->prec, g->links, in->nrhs,
-                            plqcd::kTuneOpHop};
+   const plqcd_hip_lattice *lat = g->lat;
    struct Cand {
       int order, group;
    };
@@ -51,6 +52,28 @@ extern "C" int plqcd_hip_autotune(const plqcd_hip_gauge *g, const plqcd_hip_fiel
             }
       }
    }
+   return cands;
+}
+
+void shape_out(const HopTuning &c, int best[7])
+{
+   if (!best) return;
+   const int v[7] = {c.order, c.group, c.block, c.tmarch, c.rb, c.ty, c.tz};
+   for (int k = 0; k < 7; k++) best[k] = v[k];
+}
+
+}  // namespace
+
+extern "C" int plqcd_hip_autotune(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip_field *out,
+                                  int reps, int best[7], double *best_ms)
+{
+   plqcd::clear_error();
+   if (!g || !in || !out || reps < 1) return plqcd::set_error(1, "autotune: bad arguments");
+   plqcd_hip_lattice *lat = g->lat;
+   // the shapes of plain H applications; the solver's launches keep their own
+   const plqcd::TuneKey key{{lat->L[0], lat->L[1], lat->L[2], lat->L[3]}, g->prec, g->links, in->nrhs,
+                            plqcd::kTuneOpHop};
+   const std::vector<HopTuning> cands = candidates(g, in);
    HopTuning prev{};
    const bool had = plqcd::lookup_tuning(key, &prev);
    double best_t = 1e300;
@@ -83,15 +106,7 @@ extern "C" int plqcd_hip_autotune(const plqcd_hip_gauge *g, const plqcd_hip_fiel
       return rc;
    }
    plqcd::store_tuning(key, best_c);
-   if (best) {
-      best[0] = best_c.order;
-      best[1] = best_c.group;
-      best[2] = best_c.block;
-      best[3] = best_c.tmarch;
-      best[4] = best_c.rb;
-      best[5] = best_c.ty;
-      best[6] = best_c.tz;
-   }
+   shape_out(best_c, best);
    if (best_ms) *best_ms = best_t;
    return 0;
 }
@@ -115,5 +130,99 @@ extern "C" int plqcd_hip_autotune_lookup(const plqcd_hip_gauge *g, const plqcd_h
 extern "C" int plqcd_hip_autotune_clear(void)
 {
    plqcd::clear_tuning(nullptr);
+   return 0;
+}
+
+extern "C" int plqcd_hip_autotune_solver(const plqcd_hip_gauge *g, const plqcd_hip_field *in, int reps, int best[7],
+                                         double *best_ms)
+{
+   plqcd::clear_error();
+   if (!g || !in || reps < 1 || in->lat != g->lat || in->prec != g->prec)
+      return plqcd::set_error(1, "autotune_solver: bad arguments");
+   plqcd_hip_lattice *lat = g->lat;
+   const int R = in->nrhs;
+   const plqcd::TuneKey key{{lat->L[0], lat->L[1], lat->L[2], lat->L[3]}, g->prec, g->links, R,
+                            plqcd::kTuneOpSolver};
+   // scratch: q = M p, the residual r and iterate x of the fused update, kappa
+   // and alpha per RHS, and the chunk sums; `in` plays p and is only read
+   plqcd_hip_field *q = nullptr, *r = nullptr, *x = nullptr;
+   double *dk = nullptr, *partial = nullptr;
+   int rc = plqcd_hip_field_create(lat, in->prec, R, &q);
+   if (!rc) rc = plqcd_hip_field_create(lat, in->prec, R, &r);
+   if (!rc) rc = plqcd_hip_field_create(lat, in->prec, R, &x);
+   if (!rc) {
+      std::vector<double> ka(2 * size_t(R), 0.0);  // kappa 0.124 (a typical Wilson value), alpha 0
+      for (int k = 0; k < R; k++) ka[k] = 0.124;
+      hipError_t e = hipMalloc(&dk, ka.size() * sizeof(double));
+      if (e == hipSuccess) e = hipMalloc(&partial, plqcd::partial_doubles(in) * sizeof(double));
+      if (e == hipSuccess) e = hipMemcpy(dk, ka.data(), ka.size() * sizeof(double), hipMemcpyHostToDevice);
+      if (e == hipSuccess) e = hipMemset(r->data, 0, r->bytes);
+      if (e == hipSuccess) e = hipMemset(x->data, 0, x->bytes);
+      if (e != hipSuccess) rc = plqcd::set_error(10, "autotune_solver: %s", hipGetErrorString(e));
+   }
+   plqcd::DslashOp m;  // q = M p and |q|^2
+   m.kappa = dk;
+   m.partial = partial;
+   plqcd::DslashOp md;  // M^+ q fused with x += alpha p, r -= alpha M^+ q and |r|^2
+   md.kappa = dk;
+   md.dag = 1;
+   md.partial = partial;
+   md.alpha = dk + R;
+   md.fuse_p = in;
+   md.fuse_x = x;
+   auto pair = [&]() {
+      int c = plqcd::dslash(g, in, q, PLQCD_PARITY_ALL, PLQCD_HOP_WILSON, m);
+      if (!c) c = plqcd::dslash(g, q, r, PLQCD_PARITY_ALL, PLQCD_HOP_WILSON, md);
+      return c;
+   };
+   HopTuning prev{};
+   const bool had = plqcd::lookup_tuning(key, &prev);
+   double best_t = 1e300;
+   HopTuning best_c = plqcd::default_tuning(g->prec, g->links, R, true);
+   hipEvent_t e0 = nullptr, e1 = nullptr;
+   if (!rc && hipEventCreate(&e0) != hipSuccess) rc = plqcd::set_error(10, "autotune_solver: hipEventCreate");
+   if (!rc && hipEventCreate(&e1) != hipSuccess) rc = plqcd::set_error(10, "autotune_solver: hipEventCreate");
+   if (!rc) {
+      for (const HopTuning &c : candidates(g, in)) {
+         plqcd::store_tuning(key, c);
+         rc = pair();  // warm-up
+         if (rc == 2) {  // no kernel of this shape for the fused epilogues: not a candidate
+            rc = 0;
+            continue;
+         }
+         if (rc) break;
+         (void)hipEventRecord(e0, lat->stream);
+         for (int k = 0; k < reps && !rc; k++) rc = pair();
+         (void)hipEventRecord(e1, lat->stream);
+         if (rc) break;
+         (void)hipEventSynchronize(e1);
+         float ms = 0.f;
+         (void)hipEventElapsedTime(&ms, e0, e1);
+         if (ms / reps < best_t) {
+            best_t = ms / reps;
+            best_c = c;
+         }
+      }
+      if (!rc && best_t == 1e300) rc = plqcd::set_error(2, "autotune_solver: no launch shape applies");
+   }
+   if (!rc) {
+      const hipError_t e = hipStreamSynchronize(lat->stream);
+      if (e != hipSuccess) rc = plqcd::set_error(10, "autotune_solver: %s", hipGetErrorString(e));
+   }
+   if (e0) (void)hipEventDestroy(e0);
+   if (e1) (void)hipEventDestroy(e1);
+   if (dk) (void)hipFree(dk);
+   if (partial) (void)hipFree(partial);
+   plqcd_hip_field_destroy(q);
+   plqcd_hip_field_destroy(r);
+   plqcd_hip_field_destroy(x);
+   if (rc) {
+      if (had) plqcd::store_tuning(key, prev);
+      else plqcd::clear_tuning(&key);
+      return rc;
+   }
+   plqcd::store_tuning(key, best_c);
+   shape_out(best_c, best);
+   if (best_ms) *best_ms = best_t;
    return 0;
 }

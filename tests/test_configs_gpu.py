@@ -236,3 +236,52 @@ def test_autotune_bitwise_cached_and_solver_untouched(hp):
     finally:
         lib.plqcd_hip_autotune_clear()
         _close(x, b, fo, fi, g64, g, lat)
+
+
+def test_autotune_solver_bitwise_and_cached_apart(hp):
+    """The solver tuner on 4 RHS with FP16 / 18-real sloppy fields: the tuned
+    shape is cached under the solver operation only, and a mixed-precision CG
+    afterwards takes the same iterations to the same bits."""
+    from plqcd_amd import capi
+    from plqcd_amd import solver as sv
+    lib = capi.lib()
+    L = (8, 8, 8, 16)
+    R = 4
+    V = int(np.prod(L))
+    u = hp.random_gauge_su3(V, 61)
+    lat = hp.Lattice(L)
+    g64 = hp.Gauge(lat).upload(u)
+    g16 = hp.Gauge(lat, 2, 18).upload(u)
+    b, x = hp.Field(lat, 0, R), hp.Field(lat, 0, R)
+    fs = hp.Field(lat, 2, R)
+    shape = (C.c_int * 7)()
+    best = (C.c_int * 7)()
+    ms = C.c_double()
+    try:
+        for r in range(R):
+            b.upload(hp.random_spinor(V, 62 + r), r)
+        assert lib.plqcd_hip_autotune_clear() == 0
+        kap = [KAPPA, 0.12, 0.11, 0.1]
+        st0 = sv.cg_solve(g64, b, x, kap, 1e-10, sloppy=g16)
+        x0 = [x.download(r) for r in range(R)]
+
+        assert lib.plqcd_hip_autotune_solver(g16.h, fs.h, 2, best, C.byref(ms)) == 0, capi.last_error()
+        assert ms.value > 0
+        assert lib.plqcd_hip_autotune_lookup(g16.h, fs.h, 1, shape) == 1
+        assert list(shape) == list(best)
+        assert lib.plqcd_hip_autotune_lookup(g16.h, fs.h, 0, shape) == 0  # plain H untouched
+
+        st1 = sv.cg_solve(g64, b, x, kap, 1e-10, sloppy=g16)
+        for s0, s1 in zip(st0, st1):
+            assert s1["converged"] and s1["iterations"] == s0["iterations"]
+            assert s1["true_residual"] == s0["true_residual"]
+        for r in range(R):
+            assert np.array_equal(x.download(r), x0[r])
+
+        # bad arguments (a field of another precision than the gauge) leave the entry
+        assert lib.plqcd_hip_autotune_solver(g16.h, b.h, 1, None, None) != 0
+        assert lib.plqcd_hip_autotune_lookup(g16.h, fs.h, 1, shape) == 1
+        assert list(shape) == list(best)
+    finally:
+        lib.plqcd_hip_autotune_clear()
+        _close(fs, x, b, g16, g64, lat)
