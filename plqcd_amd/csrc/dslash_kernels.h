@@ -631,7 +631,7 @@ struct BatchList {
    BatchItem<SIO, GIO> it[kBatchItems];
 };
 
-// t-march variant (wilson mode, full lattice): a thread owns one (x,y,z)
+// t-march variant (full lattice, either mode): a thread owns one (x,y,z)
 // column and walks T consecutive t-slices.  Along t each spinor and each U_t
 // is loaded once: psi(t+1), read as the forward t-neighbour at step t, is kept
 // (it is the centre site of step t+1) and its projection, multiplied by
@@ -640,7 +640,7 @@ struct BatchList {
 // as the gather kernel does.  The site parity alternates from step to step;
 // the index of the site inside its parity advances by S = L0/2*L1*L2 per
 // t-slice.  Terms are accumulated in the same order as in hop_kernel.
-template <bool DAG, int EPI, bool NORM, int BLOCK, class SIO, class GIO, class SO>
+template <int MODE, bool DAG, int EPI, bool NORM, int BLOCK, class SIO, class GIO, class SO>
 __device__ __forceinline__ void hop_tmarch_block(const SIO &in, const SO &out, const GIO &g, const HopArgs &a, int lb,
                                                  const CgFuse<SIO> &fuse)
 {
@@ -752,11 +752,11 @@ __device__ __forceinline__ void hop_tmarch_block(const SIO &in, const SO &out, c
          C acc[12];
 #pragma unroll
          for (int m = 0; m < 12; m++) acc[m] = C{0, 0};
-         hop_fwd<0, PLQCD_HOP_WILSON, DAG>(si, g, p, j, cu[0] + base, acc);
+         hop_fwd<0, MODE, DAG>(si, g, p, j, cu[0] + base, acc);
          hop_bwd<0, DAG>(si, g, p, cd[0] + base, acc);
-         hop_fwd<1, PLQCD_HOP_WILSON, DAG>(si, g, p, j, cu[1] + base, acc);
+         hop_fwd<1, MODE, DAG>(si, g, p, j, cu[1] + base, acc);
          hop_bwd<1, DAG>(si, g, p, cd[1] + base, acc);
-         hop_fwd<2, PLQCD_HOP_WILSON, DAG>(si, g, p, j, cu[2] + base, acc);
+         hop_fwd<2, MODE, DAG>(si, g, p, j, cu[2] + base, acc);
          hop_bwd<2, DAG>(si, g, p, cd[2] + base, acc);
          Sp f, cs;
          {
@@ -768,10 +768,17 @@ __device__ __forceinline__ void hop_tmarch_block(const SIO &in, const SO &out, c
             // backward t term from registers and no other block reads it: a
             // streaming load (FP16 with one RHS 6-9 % faster at 32^3 x 64,
             // 12 RHS under 1 % slower; choosing by R at run time cost the
-            // one-RHS kernel its gain)
+            // one-RHS kernel its gain).  Compat mode's forward term takes the
+            // even site's link with the same index, a second load.
             g.template load_u<true>(p, 3, j, u);
-            term_u<3, SF>(f, u, rr);
-            accum<3, SF, true>(acc, rr);
+            if constexpr (MODE == MODE_COMPAT) {
+               typename GIO::U uf;
+               g.load_u(0, 3, j, uf);
+               term_u<3, SF>(f, uf, rr);
+            } else {
+               term_u<3, SF>(f, u, rr);
+            }
+            accum<3, SF, MODE != MODE_COMPAT>(acc, rr);
             get_b(bt);
             accum<3, -SF, true>(acc, bt);
             get_cen(cs);
@@ -792,10 +799,10 @@ __device__ __forceinline__ void hop_tmarch_block(const SIO &in, const SO &out, c
    }
 }
 
-template <bool DAG, int EPI, bool NORM, int BLOCK, class SIO, class GIO, class SO>
+template <int MODE, bool DAG, int EPI, bool NORM, int BLOCK, class SIO, class GIO, class SO>
 __global__ __launch_bounds__(BLOCK) void hop_tmarch_kernel(SIO in, SO out, GIO g, HopArgs a, CgFuse<SIO> fuse)
 {
-   hop_tmarch_block<DAG, EPI, NORM, BLOCK>(in, out, g, a, logical_block(a), fuse);
+   hop_tmarch_block<MODE, DAG, EPI, NORM, BLOCK>(in, out, g, a, logical_block(a), fuse);
 }
 
 // One launch over a ragged batch (see BatchItem): each item runs t-march or
@@ -818,11 +825,11 @@ __global__ __launch_bounds__(BLOCK) void hop_batch_kernel(const BatchList<SIO, G
       int it = 0;
       while (it + 1 < nitems && b >= items.it[it + 1].first) it++;
       const BatchItem<SIO, GIO> &I = items.it[it];
-      if constexpr (MODE == PLQCD_HOP_WILSON && KIND == 1) {
-         hop_tmarch_block<false, 0, false, BLOCK>(I.in, I.out, I.g, I.a, b - I.first, CgFuse<SIO>{});
-      } else if constexpr (MODE == PLQCD_HOP_WILSON && KIND == 2) {
+      if constexpr (KIND == 1) {
+         hop_tmarch_block<MODE, false, 0, false, BLOCK>(I.in, I.out, I.g, I.a, b - I.first, CgFuse<SIO>{});
+      } else if constexpr (KIND == 2) {
          if (I.march)
-            hop_tmarch_block<false, 0, false, BLOCK>(I.in, I.out, I.g, I.a, b - I.first, CgFuse<SIO>{});
+            hop_tmarch_block<MODE, false, 0, false, BLOCK>(I.in, I.out, I.g, I.a, b - I.first, CgFuse<SIO>{});
          else
             hop_block<MODE, false, 0, false, BLOCK>(I.in, I.out, I.g, I.a, b - I.first, CgFuse<SIO>{});
       } else {
@@ -851,15 +858,18 @@ constexpr size_t tile_lds_bytes()
 // as hop_fwd + hop_bwd.  With XSH (x direction only) the lane's forward link
 // is streamed from memory and published in ux, and the backward link -- the
 // forward link of the x - 1 lane, slot sd -- is read from there after a
-// barrier; every lane of the block calls this.
-template <int MU, bool DAG, int BLOCK, bool XSH, class SIO, class GIO>
+// barrier; every lane of the block calls this.  In compat mode the forward
+// link is the even site's with the same index j (slot sx in the tile): with
+// XSH an odd lane reads it from ux after the barrier, otherwise from memory.
+template <int MU, int MODE, bool DAG, int BLOCK, bool XSH, class SIO, class GIO>
 __device__ __forceinline__ void tile_dir(const SIO &in, const GIO &g, const typename SIO::Sp::E (*buf)[BLOCK], int p,
-                                         int j, int base, int cu, int cd, int su, int sd, typename SIO::C acc[12],
-                                         typename GIO::U::E (*ux)[BLOCK])
+                                         int j, int base, int cu, int cd, int su, int sd, int sx,
+                                         typename SIO::C acc[12], typename GIO::U::E (*ux)[BLOCK])
 {
    typedef typename SIO::Sp Sp;
    constexpr int SG = DAG ? 1 : -1;
    constexpr bool share = MU == 0 && XSH;
+   constexpr bool compat = MODE == MODE_COMPAT;
    Sp f;
    typename GIO::U u;
    typename SIO::Hv r;
@@ -873,11 +883,18 @@ __device__ __forceinline__ void tile_dir(const SIO &in, const GIO &g, const type
       g.template load_u<true>(p, MU, j, u);
 #pragma unroll
       for (int k = 0; k < 9; k++) ux[k][threadIdx.x] = u.e[k];
+      if constexpr (compat) {
+         __syncthreads();
+         if (p) {
+#pragma unroll
+            for (int k = 0; k < 9; k++) u.e[k] = ux[k][sx];
+         }
+      }
    } else {
-      g.load_u(p, MU, j, u);
+      g.load_u(compat ? 0 : p, MU, j, u);
    }
    term_u<MU, SG>(f, u, r);
-   accum<MU, SG, true>(acc, r);
+   accum<MU, SG, !compat>(acc, r);
    if (sd >= 0) {
 #pragma unroll
       for (int m = 0; m < Sp::N; m++) f.e[m] = buf[m][sd];
@@ -885,7 +902,7 @@ __device__ __forceinline__ void tile_dir(const SIO &in, const GIO &g, const type
       in.load_sp(1 - p, cd + base, f);
    }
    if constexpr (share) {
-      __syncthreads();
+      if constexpr (!compat) __syncthreads();
 #pragma unroll
       for (int k = 0; k < 9; k++) u.e[k] = ux[k][sd];
    } else {
@@ -895,7 +912,7 @@ __device__ __forceinline__ void tile_dir(const SIO &in, const GIO &g, const type
    accum<MU, -SG, true>(acc, r);
 }
 
-// LDS-tiled t-march (wilson mode, full lattice, one RHS).  A block owns the
+// LDS-tiled t-march (full lattice, one RHS).  A block owns the
 // columns of a tile of Lxh x ty x tz (x, y, z) sites -- whole x rows -- for
 // both parities: lanes q = 0 and 1 of a column hold its two x-adjacent sites,
 // and the block walks T t-slices.  The centre spinors of a t-slice are in LDS
@@ -906,8 +923,10 @@ __device__ __forceinline__ void tile_dir(const SIO &in, const GIO &g, const type
 // handled as in hop_tmarch_kernel, except that FP64 blocks of up to 128 lanes
 // pass the x links between the two lanes that use each one through LDS (a
 // second barrier per slice).  Terms are accumulated in the order of hop_kernel.
-// BLOCK = 2 Lxh ty tz lanes.
-template <bool DAG, int EPI, bool NORM, int BLOCK, class SIO, class GIO>
+// Compat mode takes the forward links of odd sites from the even site with
+// the same index (its t link with a second load, since the true one still
+// serves the next slice's backward term).  BLOCK = 2 Lxh ty tz lanes.
+template <int MODE, bool DAG, int EPI, bool NORM, int BLOCK, class SIO, class GIO>
 __global__ __launch_bounds__(BLOCK) void hop_tile_kernel(SIO in, SIO out, GIO g, HopArgs a, CgFuse<SIO> fuse)
 {
    typedef typename SIO::C C;
@@ -944,6 +963,7 @@ __global__ __launch_bounds__(BLOCK) void hop_tile_kernel(SIO in, SIO out, GIO g,
    const int c = row * a.Lxh + xh;
    const int t0 = chunk * a.T;
    const int x = 2 * xh + ((q + y + z + t0) & 1);  // the same at every t-slice
+   const int sx = (1 - q) * ncol + lc;              // the lane with the same index j, other parity
    int cu[3], cd[3], su[3], sd[3];
    {
       const int oth = (1 - q) * ncol;  // lanes of the other parity
@@ -998,11 +1018,11 @@ __global__ __launch_bounds__(BLOCK) void hop_tile_kernel(SIO in, SIO out, GIO g,
       for (int m = 0; m < 12; m++) acc[m] = C{0, 0};
       // the scheduling barriers keep each direction's loads in its own
       // section, which holds the kernel within 256 VGPRs without spills
-      tile_dir<0, DAG, BLOCK, XSH>(in, g, buf, p, j, base, cu[0], cd[0], su[0], sd[0], acc, (UxRow)ux);
+      tile_dir<0, MODE, DAG, BLOCK, XSH>(in, g, buf, p, j, base, cu[0], cd[0], su[0], sd[0], sx, acc, (UxRow)ux);
       __builtin_amdgcn_sched_barrier(0);
-      tile_dir<1, DAG, BLOCK, XSH>(in, g, buf, p, j, base, cu[1], cd[1], su[1], sd[1], acc, (UxRow)ux);
+      tile_dir<1, MODE, DAG, BLOCK, XSH>(in, g, buf, p, j, base, cu[1], cd[1], su[1], sd[1], sx, acc, (UxRow)ux);
       __builtin_amdgcn_sched_barrier(0);
-      tile_dir<2, DAG, BLOCK, XSH>(in, g, buf, p, j, base, cu[2], cd[2], su[2], sd[2], acc, (UxRow)ux);
+      tile_dir<2, MODE, DAG, BLOCK, XSH>(in, g, buf, p, j, base, cu[2], cd[2], su[2], sd[2], sx, acc, (UxRow)ux);
       __builtin_amdgcn_sched_barrier(0);
       {
          Sp f;
@@ -1013,8 +1033,14 @@ __global__ __launch_bounds__(BLOCK) void hop_tile_kernel(SIO in, SIO out, GIO g,
 #pragma unroll
          for (int m = 0; m < Sp::N; m++) nxt[m][tid] = f.e[m];
          g.template load_u<true>(p, 3, j, u);  // read by this lane only (see hop_tmarch_kernel)
-         term_u<3, SF>(f, u, rr);
-         accum<3, SF, true>(acc, rr);
+         if constexpr (MODE == MODE_COMPAT) {
+            typename GIO::U uf;
+            g.load_u(0, 3, j, uf);
+            term_u<3, SF>(f, uf, rr);
+         } else {
+            term_u<3, SF>(f, u, rr);
+         }
+         accum<3, SF, MODE != MODE_COMPAT>(acc, rr);
 #pragma unroll
          for (int m = 0; m < Hv::N; m++) b.e[m] = sb[m][tid];
          accum<3, -SF, true>(acc, b);
@@ -1055,15 +1081,14 @@ int launch_one(int variant, dim3 grid, dim3 block, hipStream_t st, S si, SO so, 
 {
    constexpr bool same = std::is_same<S, SO>::value && EPI != 3;  // the tile and RHS-blocked kernels
    if (variant == V_MARCH) {
-      if constexpr (MODE == PLQCD_HOP_WILSON) {
-         hipLaunchKernelGGL((hop_tmarch_kernel<DAG, EPI, NORM, BLOCK, S, G, SO>), grid, block, 0, st, si, so, gg, a, f);
-         return 0;
-      }
+      hipLaunchKernelGGL((hop_tmarch_kernel<MODE, DAG, EPI, NORM, BLOCK, S, G, SO>), grid, block, 0, st, si, so, gg, a,
+                         f);
+      return 0;
    } else if (variant == V_TILE) {
       // two LDS buffers of BLOCK spinors, BLOCK half-spinors and (FP64 blocks
       // of up to 128 lanes) BLOCK x links must fit the CU's 160 KiB
-      if constexpr (same && MODE == PLQCD_HOP_WILSON && tile_lds_bytes<S, G, BLOCK>() <= 136 * 1024) {
-         hipLaunchKernelGGL((hop_tile_kernel<DAG, EPI, NORM, BLOCK, S, G>), grid, block, 0, st, si, so, gg, a, f);
+      if constexpr (same && tile_lds_bytes<S, G, BLOCK>() <= 136 * 1024) {
+         hipLaunchKernelGGL((hop_tile_kernel<MODE, DAG, EPI, NORM, BLOCK, S, G>), grid, block, 0, st, si, so, gg, a, f);
          return 0;
       }
    } else if (variant == V_RB2) {
@@ -1150,11 +1175,10 @@ int launch_t(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip_fiel
    }
    if (norm && a.nparity != 2) return plqcd::set_error(2, "hop: norms are only fused into full-lattice launches");
    const int T = tu.tmarch;
-   // the t-march kernel serves full-lattice wilson applications whose time
-   // extent is a multiple of T (and, with a norm, whose t-slices hold whole
-   // reduction chunks)
-   const bool march = rb == 1 && T > 0 && mode == PLQCD_HOP_WILSON && a.nparity == 2 && lat->L[3] % T == 0 &&
-                      (!norm || a.S % REDUCE_CHUNK == 0);
+   // the t-march kernel serves full-lattice applications (either mode) whose
+   // time extent is a multiple of T (and, with a norm, whose t-slices hold
+   // whole reduction chunks)
+   const bool march = rb == 1 && T > 0 && a.nparity == 2 && lat->L[3] % T == 0 && (!norm || a.S % REDUCE_CHUNK == 0);
    // the LDS-tiled t-march: one RHS, a tile of whole x rows dividing the y and
    // z extents, 2 Lxh ty tz lanes per block (at most 256 with F64 storage for
    // the LDS), and, with a norm, reduction chunks that do not leave a tile row
@@ -1249,10 +1273,10 @@ int launch_batch_t(int n, const plqcd_hip_gauge *const *g, const plqcd_hip_field
          I.a = make_hop_args(lat, PLQCD_PARITY_ALL);
          I.a.R = in[i0 + i]->nrhs;
          I.a.spb = REDUCE_CHUNK * (block / (REDUCE_CHUNK * I.a.R));
-         // wilson items march along t with the precision's t-slice count when it
+         // items march along t with the precision's t-slice count when it
          // divides the time extent; the others use gather blocks
          const int T = plqcd::default_tuning(PREC, LINKS, I.a.R).tmarch;
-         I.march = mode == PLQCD_HOP_WILSON && T > 0 && lat->L[3] % T == 0;
+         I.march = T > 0 && lat->L[3] % T == 0;
          I.a.T = I.march ? T : 1;
          I.nb = I.march ? 2 * ((I.a.S + I.a.spb - 1) / I.a.spb) * (lat->L[3] / T)
                         : 2 * ((I.a.Vh + I.a.spb - 1) / I.a.spb);
@@ -1269,10 +1293,17 @@ int launch_batch_t(int n, const plqcd_hip_gauge *const *g, const plqcd_hip_field
          const int span = ((total + 8 * kBatchGroup - 1) / (8 * kBatchGroup)) * kBatchGroup;
          hipLaunchKernelGGL(kern, dim3(8 * span), dim3(block), 0, st, L, m, total);
       };
-      if (mode == PLQCD_HOP_COMPAT) go(hop_batch_kernel<PLQCD_HOP_COMPAT, 0, 256, S, G>);
-      else if (nmarch == 0) go(hop_batch_kernel<PLQCD_HOP_WILSON, 0, 256, S, G>);
-      else if (nmarch == m) go(hop_batch_kernel<PLQCD_HOP_WILSON, 1, 256, S, G>);
-      else go(hop_batch_kernel<PLQCD_HOP_WILSON, 2, 256, S, G>);
+      if (mode == PLQCD_HOP_COMPAT) {
+         if (nmarch == 0) go(hop_batch_kernel<PLQCD_HOP_COMPAT, 0, 256, S, G>);
+         else if (nmarch == m) go(hop_batch_kernel<PLQCD_HOP_COMPAT, 1, 256, S, G>);
+         else go(hop_batch_kernel<PLQCD_HOP_COMPAT, 2, 256, S, G>);
+      } else if (nmarch == 0) {
+         go(hop_batch_kernel<PLQCD_HOP_WILSON, 0, 256, S, G>);
+      } else if (nmarch == m) {
+         go(hop_batch_kernel<PLQCD_HOP_WILSON, 1, 256, S, G>);
+      } else {
+         go(hop_batch_kernel<PLQCD_HOP_WILSON, 2, 256, S, G>);
+      }
       const hipError_t e = hipGetLastError();
       if (e != hipSuccess) return plqcd::set_error(10, "hop_batch: %s", hipGetErrorString(e));
    }

@@ -317,30 +317,31 @@ TILES = [((8, 8, 8, 16), [(4, 4), (8, 2), (2, 8), (8, 4), (4, 8)]),
 def test_tile_kernel_bitwise_equal_tmarch(hp, orc, L, tiles, prec, links, tol):
     """The LDS-tiled t-march gives bitwise the t-march's and the gather
     kernel's output for every tile (including tiles spanning a whole extent,
-    whose periodic neighbours lie inside the tile), and matches the oracle."""
+    whose periodic neighbours lie inside the tile), in both modes, and matches
+    the oracle."""
     from plqcd_amd import capi
     q, u = inputs(hp, L, 61, 62)
-    ref = orc.hopping_full(q, u, L, orc.MODE_WILSON)
+    refs = {hp.HOP_WILSON: orc.hopping_full(q, u, L, orc.MODE_WILSON),
+            hp.HOP_COMPAT: orc.hopping_full(q, u, L, orc.MODE_COMPAT)}
     try:
-        _set_tuning(prec, (1, 1, 256, 0, 1, 0, 0))
-        gather = device_hop(hp, L, q, u, hp.HOP_WILSON, prec=prec, links=links)
-        assert site_rel_err(gather, ref) < tol
-        for T in (4, 8):
-            _set_tuning(prec, (1, 1, 256, T, 1, 0, 0))
-            assert np.array_equal(device_hop(hp, L, q, u, hp.HOP_WILSON, prec=prec, links=links), gather), T
-            for ty, tz in tiles:
-                for order in (0, 1):
-                    _set_tuning(prec, (order, 1, 256, T, 1, ty, tz))
-                    out = device_hop(hp, L, q, u, hp.HOP_WILSON, prec=prec, links=links)
-                    assert np.array_equal(out, gather), (T, ty, tz, order)
-        # the half-lattice and compat launches ignore the tile
-        _set_tuning(prec, (1, 1, 256, 4, 1, 4, 4))
-        for parity in (hp.PARITY_EVEN, hp.PARITY_ODD):
-            half = device_hop(hp, L, q, u, hp.HOP_WILSON, parity=parity, prec=prec, links=links)
-            sl = slice(0, half.shape[0] // 2) if parity == hp.PARITY_EVEN else slice(half.shape[0] // 2, None)
-            assert np.array_equal(half[sl], gather[sl])
-        comp = device_hop(hp, L, q, u, hp.HOP_COMPAT, prec=prec, links=links)
-        assert site_rel_err(comp, orc.hopping_full(q, u, L, orc.MODE_COMPAT)) < tol
+        for mode, ref in refs.items():
+            _set_tuning(prec, (1, 1, 256, 0, 1, 0, 0))
+            gather = device_hop(hp, L, q, u, mode, prec=prec, links=links)
+            assert site_rel_err(gather, ref) < tol, mode
+            for T in (4, 8):
+                _set_tuning(prec, (1, 1, 256, T, 1, 0, 0))
+                assert np.array_equal(device_hop(hp, L, q, u, mode, prec=prec, links=links), gather), (mode, T)
+                for ty, tz in tiles:
+                    for order in (0, 1):
+                        _set_tuning(prec, (order, 1, 256, T, 1, ty, tz))
+                        out = device_hop(hp, L, q, u, mode, prec=prec, links=links)
+                        assert np.array_equal(out, gather), (mode, T, ty, tz, order)
+            # the half-lattice launches ignore the tile
+            _set_tuning(prec, (1, 1, 256, 4, 1, 4, 4))
+            for parity in (hp.PARITY_EVEN, hp.PARITY_ODD):
+                half = device_hop(hp, L, q, u, mode, parity=parity, prec=prec, links=links)
+                sl = slice(0, half.shape[0] // 2) if parity == hp.PARITY_EVEN else slice(half.shape[0] // 2, None)
+                assert np.array_equal(half[sl], gather[sl]), (mode, parity)
     finally:
         assert capi.lib().plqcd_hip_hop_tuning_reset(prec) == 0
 
