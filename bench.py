@@ -15,8 +15,9 @@ boundaries, Eq. 2.1 (wilson mode).  Prints one JSON line.
   formats   = the same H in every storage format: FP64 12-real links, FP32
               and FP16 with 18/12-real links, FP64 18-real (200 launches each)
   compat    = the reference's own operator (compat mode), device-resident
-  dropin    = wall time per call of the drop-in eo+oe on host arrays
-              (transfers included)
+  dropin    = wall time per call of the drop-in eo then oe on host arrays
+              (transfers included), links uploaded per call or cached, and
+              of the fused call
   solver    = plain FP64 CG on one RHS against the mixed-precision CG on 12
               RHS (FP16 storage, FP64 reliable updates), true residual 1e-10
   cg_48c96  = the same comparison at 48^3 x 96, one RHS each
@@ -210,24 +211,43 @@ def compat_run(L, steps=200, warmup=10):
 
 
 def dropin_run(L, calls=5):
-    """Wall seconds per call of the drop-in plqcd_hopping_matrix_hip (eo then
-    oe on host arrays in the reference layout, compat mode): the upload of
-    qin and the links, both halves of H and the download of qout
-    (tests/test_hopping.c:287-301 times the same eo+oe pair on host arrays)."""
-    from plqcd_amd import hopping as hp
+    """Wall seconds per call of the drop-in entry points on host arrays in the
+    reference layout, compat mode (tests/test_hopping.c:287-301 times the eo
+    then oe pair on host arrays): the pair as the harness calls it, with the
+    links uploaded on every call (the default) and with the link cache on
+    (plqcd_hip_cache_gauge: uploaded once, reused while the pointer is the
+    same), and the fused plqcd_hopping_matrix_hip.  Each call includes its
+    transfers."""
+    from plqcd_amd import capi, hopping as hp
     V = int(np.prod(L))
     u = hp.random_gauge_su3(V, 1)
     q = hp.random_spinor(V, 2)
     out = hp.spinor_field(V)
     hp.init_dims(L)
     hp.set_mode(hp.HOP_COMPAT)
-    hp.hopping_matrix(q, out, u)  # device buffers and first-touch costs
-    ts = [hp.hopping_matrix(q, out, u) for _ in range(calls)]
+    lib = capi.lib()
+
+    def pair():
+        return hp.hopping_matrix_eo(q, out, u) + hp.hopping_matrix_oe(q, out, u)
+
+    def timed(fn):
+        fn()  # device buffers and first-touch costs
+        return float(np.median([fn() for _ in range(calls)]))
+
+    t_pair = timed(pair)
+    lib.plqcd_hip_cache_gauge(1)
+    t_cached = timed(pair)
+    lib.plqcd_hip_cache_gauge(0)
+    t_fused = timed(lambda: hp.hopping_matrix(q, out, u))
     hp.finalize_plqcd()
-    t = float(np.median(ts))
-    host_bytes = V * (192 + 576 + 192)
-    return {"calls": calls, "seconds_per_call": round(t, 4), "gflops": round(FLOP_PER_SITE * V / t / 1e9, 2),
-            "host_bytes_per_call": host_bytes, "host_GBps": round(host_bytes / t / 1e9, 2)}
+    spin, links = V * 192, V * 576
+    rec = {"calls": calls}
+    for key, t, host_bytes in (("eo_oe_pair", t_pair, 2 * links + 2 * spin),
+                               ("eo_oe_pair_links_cached", t_cached, 2 * spin),
+                               ("fused", t_fused, links + 2 * spin)):
+        rec[key] = {"seconds_per_call": round(t, 4), "gflops": round(FLOP_PER_SITE * V / t / 1e9, 2),
+                    "host_bytes_per_call": host_bytes, "host_GBps": round(host_bytes / t / 1e9, 2)}
+    return rec
 
 
 def cg48_compare(kappa=0.124, tol=1e-10):

@@ -15,7 +15,7 @@
 
 namespace {
 
-constexpr size_t kStageBytes = size_t(64) << 20;  // staging chunk for host transfers
+constexpr size_t kXferBytes = size_t(32) << 20;  // one chunk of a host <-> device transfer
 constexpr int kPackBlock = 256;
 
 // --------------------------------------------------------- spinor repacking
@@ -111,11 +111,112 @@ __global__ void pack_gauge(const d2 *__restrict__ aos, void *dst, int prec, int 
    }
 }
 
-int ensure_stage(plqcd_hip_lattice *lat)
+#ifndef MADV_POPULATE_WRITE
+#define MADV_POPULATE_WRITE 23
+#endif
+// Maps the whole pages of a host destination in one call before they are
+// written: the first write to a fresh page otherwise takes a fault and a
+// zeroing per page (a fresh 2 GB array filled at ~1.5-3.5 GB/s instead of
+// ~9 GB/s).  The data is not touched; kernels without the call return an
+// error, which changes nothing.
+void prefault(void *p, size_t n)
 {
-   if (lat->stage) return 0;
-   PLQCD_HIP_TRY(hipMalloc(&lat->stage, kStageBytes));
-   lat->stage_bytes = kStageBytes;
+   const uintptr_t pg = 4096;
+   const uintptr_t a = (reinterpret_cast<uintptr_t>(p) + pg - 1) & ~(pg - 1);
+   const uintptr_t b = (reinterpret_cast<uintptr_t>(p) + n) & ~(pg - 1);
+   if (b > a) (void)madvise(reinterpret_cast<void *>(a), b - a, MADV_POPULATE_WRITE);
+}
+
+template <class F>
+void dispatch_prec(int prec, F f)
+{
+   switch (prec) {
+      case PLQCD_PREC_F64: f(std::integral_constant<int, PLQCD_PREC_F64>{}); break;
+      case PLQCD_PREC_F32: f(std::integral_constant<int, PLQCD_PREC_F32>{}); break;
+      default: f(std::integral_constant<int, PLQCD_PREC_F16>{});
+   }
+}
+
+int ensure_xfer(plqcd_hip_lattice *lat)
+{
+   for (int b = 0; b < 2; b++) {
+      if (!lat->xdev[b]) PLQCD_HIP_TRY(hipMalloc(&lat->xdev[b], kXferBytes));
+      if (!lat->xpin[b]) PLQCD_HIP_TRY(hipHostMalloc(&lat->xpin[b], kXferBytes, hipHostMallocDefault));
+      if (!lat->xev[b]) PLQCD_HIP_TRY(hipEventCreateWithFlags(&lat->xev[b], hipEventDisableTiming));
+   }
+   return 0;
+}
+
+// Host -> device transfer of `count` elements of `esize` bytes, from element
+// `first` of `host`, in chunks that alternate between the lattice's two
+// buffer pairs.  Each chunk is copied into a page-locked buffer (by several
+// threads) while the previous one is in DMA and being repacked, or goes by
+// DMA alone when the host range is page-locked (plqcd_hip_host_alloc).
+// pack(device chunk, first element, elements) enqueues the repacking kernel
+// on the lattice stream.  Returns when the last chunk has been repacked.
+template <class Pack>
+int upload_chunks(plqcd_hip_lattice *lat, const void *host, size_t esize, size_t first, size_t count, Pack pack)
+{
+   if (int rc = ensure_xfer(lat)) return rc;
+   hipStream_t st = lat->stream;
+   const size_t per = kXferBytes / esize;
+   const char *h = static_cast<const char *>(host);
+   const bool pinned = plqcd::host_pinned(h + first * esize, count * esize);
+   bool used[2] = {false, false};
+   for (size_t i0 = first, c = 0; i0 < first + count; i0 += per, c++) {
+      const int b = (int)(c & 1);
+      const size_t n = std::min(per, first + count - i0);
+      const void *src = h + i0 * esize;
+      if (!pinned) {
+         if (used[b]) PLQCD_HIP_TRY(hipEventSynchronize(lat->xev[b]));  // its previous chunk has left
+         plqcd::host_copy(lat->xpin[b], src, n * esize);
+         src = lat->xpin[b];
+      }
+      PLQCD_HIP_TRY(hipMemcpyAsync(lat->xdev[b], src, n * esize, hipMemcpyHostToDevice, st));
+      pack(lat->xdev[b], i0, n);
+      PLQCD_HIP_TRY(hipGetLastError());
+      PLQCD_HIP_TRY(hipEventRecord(lat->xev[b], st));
+      used[b] = true;
+   }
+   PLQCD_HIP_TRY(hipStreamSynchronize(st));
+   return 0;
+}
+
+// Device -> host transfer, the reverse of upload_chunks: unpack(device chunk,
+// first element, elements) enqueues the kernel that writes a chunk in the host
+// layout; each chunk's DMA overlaps the copy-out of the previous one.
+template <class Unpack>
+int download_chunks(plqcd_hip_lattice *lat, void *host, size_t esize, size_t first, size_t count, Unpack unpack)
+{
+   if (int rc = ensure_xfer(lat)) return rc;
+   hipStream_t st = lat->stream;
+   const size_t per = kXferBytes / esize;
+   char *h = static_cast<char *>(host);
+   const bool pinned = plqcd::host_pinned(h + first * esize, count * esize);
+   if (!pinned) prefault(h + first * esize, count * esize);
+   size_t pend_i0 = 0, pend_n = 0;
+   int pend_b = -1;  // chunk whose DMA is in flight
+   for (size_t i0 = first, c = 0; i0 < first + count; i0 += per, c++) {
+      const int b = (int)(c & 1);
+      const size_t n = std::min(per, first + count - i0);
+      unpack(lat->xdev[b], i0, n);
+      PLQCD_HIP_TRY(hipGetLastError());
+      void *dst = pinned ? static_cast<void *>(h + i0 * esize) : lat->xpin[b];
+      PLQCD_HIP_TRY(hipMemcpyAsync(dst, lat->xdev[b], n * esize, hipMemcpyDeviceToHost, st));
+      PLQCD_HIP_TRY(hipEventRecord(lat->xev[b], st));
+      if (!pinned && pend_b >= 0) {
+         PLQCD_HIP_TRY(hipEventSynchronize(lat->xev[pend_b]));
+         plqcd::host_copy(h + pend_i0 * esize, lat->xpin[pend_b], pend_n * esize);
+      }
+      pend_i0 = i0;
+      pend_n = n;
+      pend_b = b;
+   }
+   if (!pinned && pend_b >= 0) {
+      PLQCD_HIP_TRY(hipEventSynchronize(lat->xev[pend_b]));
+      plqcd::host_copy(h + pend_i0 * esize, lat->xpin[pend_b], pend_n * esize);
+   }
+   PLQCD_HIP_TRY(hipStreamSynchronize(st));
    return 0;
 }
 
@@ -168,7 +269,11 @@ extern "C" void plqcd_hip_lattice_destroy(plqcd_hip_lattice *lat)
 {
    if (!lat) return;
    (void)hipStreamSynchronize(lat->stream);
-   if (lat->stage) (void)hipFree(lat->stage);
+   for (int b = 0; b < 2; b++) {
+      if (lat->xdev[b]) (void)hipFree(lat->xdev[b]);
+      if (lat->xpin[b]) (void)hipHostFree(lat->xpin[b]);
+      if (lat->xev[b]) (void)hipEventDestroy(lat->xev[b]);
+   }
    (void)hipEventDestroy(lat->ev_start);
    (void)hipEventDestroy(lat->ev_stop);
    (void)hipStreamDestroy(lat->own_stream);
@@ -250,18 +355,11 @@ extern "C" int plqcd_hip_gauge_upload(plqcd_hip_gauge *g, const su3 *u)
 {
    if (!g || !u) return plqcd::set_error(1, "plqcd_hip_gauge_upload: null argument");
    plqcd_hip_lattice *lat = g->lat;
-   if (int rc = ensure_stage(lat)) return rc;
-   const int nlinks = 4 * lat->V;
-   const int per = (int)(lat->stage_bytes / sizeof(su3));
-   for (int l0 = 0; l0 < nlinks; l0 += per) {
-      const int n = std::min(per, nlinks - l0);
-      PLQCD_HIP_TRY(hipMemcpyAsync(lat->stage, u + l0, (size_t)n * sizeof(su3), hipMemcpyHostToDevice, lat->stream));
-      hipLaunchKernelGGL(pack_gauge, dim3((n + kPackBlock - 1) / kPackBlock), dim3(kPackBlock), 0, lat->stream,
-                         static_cast<const d2 *>(lat->stage), g->data, g->prec, g->links, lat->Vh, l0, n);
-      PLQCD_HIP_TRY(hipGetLastError());
-   }
-   PLQCD_HIP_TRY(hipStreamSynchronize(lat->stream));
-   return 0;
+   return upload_chunks(lat, u, sizeof(su3), 0, size_t(4) * lat->V, [&](void *d, size_t l0, size_t n) {
+      hipLaunchKernelGGL(pack_gauge, dim3((unsigned)((n + kPackBlock - 1) / kPackBlock)), dim3(kPackBlock), 0,
+                         lat->stream, static_cast<const d2 *>(d), g->data, g->prec, g->links, lat->Vh, (int)l0,
+                         (int)n);
+   });
 }
 
 // =================================================================== fields
@@ -322,31 +420,14 @@ int plqcd::field_upload_range(plqcd_hip_field *f, int rhs, const spinor *host, i
    if (!host) return plqcd::set_error(1, "plqcd_hip_field_upload: null host pointer");
    plqcd_hip_lattice *lat = f->lat;
    if (first < 0 || last > lat->V || first > last) return plqcd::set_error(1, "bad site range");
-   if (int rc = ensure_stage(lat)) return rc;
-   const int per = (int)(lat->stage_bytes / sizeof(spinor));
-   for (int i0 = first; i0 < last; i0 += per) {
-      const int n = std::min(per, last - i0);
-      PLQCD_HIP_TRY(
-          hipMemcpyAsync(lat->stage, host + i0, (size_t)n * sizeof(spinor), hipMemcpyHostToDevice, lat->stream));
-      const dim3 grid((n + kPackBlock - 1) / kPackBlock), block(kPackBlock);
-      const d2 *src = static_cast<const d2 *>(lat->stage);
-      switch (f->prec) {
-         case PLQCD_PREC_F64:
-            hipLaunchKernelGGL(pack_spinor<PLQCD_PREC_F64>, grid, block, 0, lat->stream, src,
-                               spinor_io<PLQCD_PREC_F64>(f), lat->Vh, rhs, i0, n);
-            break;
-         case PLQCD_PREC_F32:
-            hipLaunchKernelGGL(pack_spinor<PLQCD_PREC_F32>, grid, block, 0, lat->stream, src,
-                               spinor_io<PLQCD_PREC_F32>(f), lat->Vh, rhs, i0, n);
-            break;
-         default:
-            hipLaunchKernelGGL(pack_spinor<PLQCD_PREC_F16>, grid, block, 0, lat->stream, src,
-                               spinor_io<PLQCD_PREC_F16>(f), lat->Vh, rhs, i0, n);
-      }
-      PLQCD_HIP_TRY(hipGetLastError());
-   }
-   PLQCD_HIP_TRY(hipStreamSynchronize(lat->stream));
-   return 0;
+   return upload_chunks(lat, host, sizeof(spinor), first, last - first, [&](void *d, size_t i0, size_t n) {
+      const dim3 grid((unsigned)((n + kPackBlock - 1) / kPackBlock)), block(kPackBlock);
+      dispatch_prec(f->prec, [&](auto P) {
+         constexpr int PP = decltype(P)::value;
+         hipLaunchKernelGGL(pack_spinor<PP>, grid, block, 0, lat->stream, static_cast<const d2 *>(d),
+                            spinor_io<PP>(f), lat->Vh, rhs, (int)i0, (int)n);
+      });
+   });
 }
 
 int plqcd::field_download_range(const plqcd_hip_field *f, int rhs, spinor *host, int first, int last)
@@ -355,45 +436,18 @@ int plqcd::field_download_range(const plqcd_hip_field *f, int rhs, spinor *host,
    if (!host) return plqcd::set_error(1, "plqcd_hip_field_download: null host pointer");
    plqcd_hip_lattice *lat = f->lat;
    if (first < 0 || last > lat->V || first > last) return plqcd::set_error(1, "bad site range");
-   if (int rc = ensure_stage(lat)) return rc;
-   const int per = (int)(lat->stage_bytes / sizeof(spinor));
-   for (int i0 = first; i0 < last; i0 += per) {
-      const int n = std::min(per, last - i0);
-      const dim3 grid((n + kPackBlock - 1) / kPackBlock), block(kPackBlock);
-      d2 *dst = static_cast<d2 *>(lat->stage);
-      switch (f->prec) {
-         case PLQCD_PREC_F64:
-            hipLaunchKernelGGL(unpack_spinor<PLQCD_PREC_F64>, grid, block, 0, lat->stream, dst,
-                               spinor_io<PLQCD_PREC_F64>(f), lat->Vh, rhs, i0, n);
-            break;
-         case PLQCD_PREC_F32:
-            hipLaunchKernelGGL(unpack_spinor<PLQCD_PREC_F32>, grid, block, 0, lat->stream, dst,
-                               spinor_io<PLQCD_PREC_F32>(f), lat->Vh, rhs, i0, n);
-            break;
-         default:
-            hipLaunchKernelGGL(unpack_spinor<PLQCD_PREC_F16>, grid, block, 0, lat->stream, dst,
-                               spinor_io<PLQCD_PREC_F16>(f), lat->Vh, rhs, i0, n);
-      }
-      PLQCD_HIP_TRY(hipGetLastError());
-      PLQCD_HIP_TRY(
-          hipMemcpyAsync(host + i0, lat->stage, (size_t)n * sizeof(spinor), hipMemcpyDeviceToHost, lat->stream));
-   }
-   PLQCD_HIP_TRY(hipStreamSynchronize(lat->stream));
-   return 0;
+   return download_chunks(lat, host, sizeof(spinor), first, last - first, [&](void *d, size_t i0, size_t n) {
+      const dim3 grid((unsigned)((n + kPackBlock - 1) / kPackBlock)), block(kPackBlock);
+      dispatch_prec(f->prec, [&](auto P) {
+         constexpr int PP = decltype(P)::value;
+         hipLaunchKernelGGL(unpack_spinor<PP>, grid, block, 0, lat->stream, static_cast<d2 *>(d), spinor_io<PP>(f),
+                            lat->Vh, rhs, (int)i0, (int)n);
+      });
+   });
 }
 
 namespace {
 constexpr size_t kStagerChunk = 32u << 20;  // bytes per pinned / device buffer
-
-template <class F>
-void dispatch_prec(int prec, F f)
-{
-   switch (prec) {
-      case PLQCD_PREC_F64: f(std::integral_constant<int, PLQCD_PREC_F64>{}); break;
-      case PLQCD_PREC_F32: f(std::integral_constant<int, PLQCD_PREC_F32>{}); break;
-      default: f(std::integral_constant<int, PLQCD_PREC_F16>{});
-   }
-}
 }  // namespace
 
 int plqcd::Stager::create(int device, Stager **out)
@@ -437,7 +491,7 @@ int plqcd::Stager::upload(plqcd_hip_field *f, int rhs, const spinor *host)
    for (int i0 = 0, c = 0; i0 < V; i0 += per, c++) {
       const int b = c & 1, n = std::min(per, V - i0);
       if (used[b]) PLQCD_HIP_TRY(hipEventSynchronize(ev_[b]));  // buffer b's previous chunk is packed
-      std::memcpy(pin_[b], host + i0, (size_t)n * sizeof(spinor));
+      plqcd::host_copy(pin_[b], host + i0, (size_t)n * sizeof(spinor));
       PLQCD_HIP_TRY(hipMemcpyAsync(dev_[b], pin_[b], (size_t)n * sizeof(spinor), hipMemcpyHostToDevice, st_));
       const d2 *src = static_cast<const d2 *>(dev_[b]);
       const dim3 grid((n + kPackBlock - 1) / kPackBlock), block(kPackBlock);
@@ -473,7 +527,7 @@ int plqcd::Stager::download(const plqcd_hip_field *f, int rhs, spinor *host)
       PLQCD_HIP_TRY(hipEventRecord(ev_[b], st_));
       if (pend_i0 >= 0) {  // copy out the previous chunk while this one is in flight
          PLQCD_HIP_TRY(hipEventSynchronize(ev_[pend_b]));
-         std::memcpy(host + pend_i0, pin_[pend_b], (size_t)pend_n * sizeof(spinor));
+         plqcd::host_copy(host + pend_i0, pin_[pend_b], (size_t)pend_n * sizeof(spinor));
       }
       pend_i0 = i0;
       pend_n = n;
@@ -481,7 +535,7 @@ int plqcd::Stager::download(const plqcd_hip_field *f, int rhs, spinor *host)
    }
    if (pend_i0 >= 0) {
       PLQCD_HIP_TRY(hipEventSynchronize(ev_[pend_b]));
-      std::memcpy(host + pend_i0, pin_[pend_b], (size_t)pend_n * sizeof(spinor));
+      plqcd::host_copy(host + pend_i0, pin_[pend_b], (size_t)pend_n * sizeof(spinor));
    }
    return 0;
 }
@@ -544,7 +598,7 @@ int plqcd::Stager::upload_raw(void *dev, const void *host, size_t bytes)
       const int b = (int)(c & 1);
       const size_t n = std::min(kStagerChunk, bytes - o);
       if (used[b]) PLQCD_HIP_TRY(hipEventSynchronize(ev_[b]));  // buffer b's previous chunk has left
-      std::memcpy(pin_[b], static_cast<const char *>(host) + o, n);
+      plqcd::host_copy(pin_[b], static_cast<const char *>(host) + o, n);
       PLQCD_HIP_TRY(hipMemcpyAsync(static_cast<char *>(dev) + o, pin_[b], n, hipMemcpyHostToDevice, st_));
       PLQCD_HIP_TRY(hipEventRecord(ev_[b], st_));
       used[b] = true;
@@ -552,24 +606,6 @@ int plqcd::Stager::upload_raw(void *dev, const void *host, size_t bytes)
    PLQCD_HIP_TRY(hipStreamSynchronize(st_));
    return 0;
 }
-
-namespace {
-#ifndef MADV_POPULATE_WRITE
-#define MADV_POPULATE_WRITE 23
-#endif
-// Maps the whole pages of a host destination in one call before they are
-// written: the first write to a fresh page otherwise takes a fault and a
-// zeroing per page (a fresh 2 GB array filled at ~1.5-3.5 GB/s instead of
-// ~9 GB/s).  The data is not touched; kernels without the call return an
-// error, which changes nothing.
-void prefault(void *p, size_t n)
-{
-   const uintptr_t pg = 4096;
-   const uintptr_t a = (reinterpret_cast<uintptr_t>(p) + pg - 1) & ~(pg - 1);
-   const uintptr_t b = (reinterpret_cast<uintptr_t>(p) + n) & ~(pg - 1);
-   if (b > a) (void)madvise(reinterpret_cast<void *>(a), b - a, MADV_POPULATE_WRITE);
-}
-}  // namespace
 
 int plqcd::Stager::download_raw(void *host, const void *dev, size_t bytes, hipEvent_t after)
 {
@@ -593,7 +629,7 @@ int plqcd::Stager::download_raw(void *host, const void *dev, size_t bytes, hipEv
       PLQCD_HIP_TRY(hipEventRecord(ev_[b], st_));
       if (pend_b >= 0) {  // copy out the previous chunk while this one is in flight
          PLQCD_HIP_TRY(hipEventSynchronize(ev_[pend_b]));
-         std::memcpy(static_cast<char *>(host) + pend_o, pin_[pend_b], pend_n);
+         plqcd::host_copy(static_cast<char *>(host) + pend_o, pin_[pend_b], pend_n);
       }
       pend_o = o;
       pend_n = n;
@@ -601,7 +637,7 @@ int plqcd::Stager::download_raw(void *host, const void *dev, size_t bytes, hipEv
    }
    if (pend_b >= 0) {
       PLQCD_HIP_TRY(hipEventSynchronize(ev_[pend_b]));
-      std::memcpy(static_cast<char *>(host) + pend_o, pin_[pend_b], pend_n);
+      plqcd::host_copy(static_cast<char *>(host) + pend_o, pin_[pend_b], pend_n);
    }
    return 0;
 }

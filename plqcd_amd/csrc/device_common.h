@@ -40,8 +40,12 @@ struct plqcd_hip_lattice {
    hipStream_t own_stream;
    hipStream_t stream;
    hipEvent_t ev_start, ev_stop;
-   void *stage;  // device staging buffer for host<->device repacking
-   size_t stage_bytes;
+   // host <-> device transfers, allocated on first use: two page-locked host
+   // buffers and two device buffers of the same size, in which consecutive
+   // chunks alternate, and an event per pair
+   void *xpin[2];
+   void *xdev[2];
+   hipEvent_t xev[2];
 };
 
 struct plqcd_hip_gauge {

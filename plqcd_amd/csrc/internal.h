@@ -21,4 +21,7 @@ int field_copy_range(const plqcd_hip_field *src, plqcd_hip_field *dst, int rhs, 
 // Releases the device state owned by the drop-in entry points (dirac_api).
 void release_dropin_state();
 
+// memcpy of host memory, split over a small pool of threads when large.
+void host_copy(void *dst, const void *src, size_t n);
+
 }  // namespace plqcd
