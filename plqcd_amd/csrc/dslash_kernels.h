@@ -682,16 +682,17 @@ __device__ __forceinline__ void hop_tmarch_block(const SIO &in, const SO &out, c
       cd[2] = rzm * a.Lxh + xh;
    }
    const int Lt = a.L[3];
-   // The lane's centre spinor and backward t term live across t-slices.  With
-   // FP64 storage and an epilogue they stay in LDS (each lane reads only its
+   // The lane's centre spinor and backward t term live across t-slices.  In
+   // the FP64 residual epilogue they stay in LDS (each lane reads only its
    // own column, so no barrier), which frees 72 registers (226-254 instead of
-   // 256 + 20-38 AGPRs) and lets two blocks share a CU: the residual epilogue
-   // at 32^3 x 64 with 12 RHS took 4.1 instead of 5.0 ms and M the same time.
-   // Plain H (no epilogue) ran 2 % slower that way and keeps them in
-   // registers, as do FP32 and FP16.
+   // 256 + 20-38 AGPRs) and lets two blocks share a CU: at 32^3 x 64 with 12
+   // RHS it took 4.1 instead of 5.0 ms.  M (EPI 1) took the same time either
+   // way but, with two blocks per CU sharing the L2, moved 2.77 instead of
+   // 1.81 times its compulsory bytes, and plain H ran 2 % slower, so they and
+   // the other epilogues keep the state in registers, as do FP32 and FP16.
    typedef typename SIO::Sp Sp;
    typedef typename SIO::Hv Hv;
-   constexpr bool CL = std::is_same<typename SIO::real, double>::value && EPI >= 1;
+   constexpr bool CL = std::is_same<typename SIO::real, double>::value && EPI == 3;
    __shared__ typename Sp::E lcen[CL ? Sp::N : 1][CL ? BLOCK : 1];
    __shared__ typename Hv::E lbt[CL ? Hv::N : 1][CL ? BLOCK : 1];
    const int tid = threadIdx.x;
