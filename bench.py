@@ -21,6 +21,12 @@ boundaries, Eq. 2.1 (wilson mode).  Prints one JSON line.
   solver    = plain FP64 CG on one RHS against the mixed-precision CG on 12
               RHS (FP16 storage, FP64 reliable updates), true residual 1e-10
   cg_48c96  = the same comparison at 48^3 x 96, one RHS each
+  multi_rhs = H on 1, 4, 12 and 24 RHS sharing one gauge field (FP64 18-real,
+              FP16 12-real), us per application and per RHS
+  ragged_batch = one launch over 16^3x32 + 24^3x48 + 32^3x64 with 12 RHS each
+              against three separate launches
+  serve     = a short Poisson serving run (--serve-seconds, 2 requests/s):
+              p50/p99 latency and solves/s beside one-at-a-time FP64 solves
 The headline is timed last, after the other GPU measurements, so that its
 warm-up launches start on a GPU at its steady clocks.
 """
@@ -348,6 +354,83 @@ def time_hop(hp, g, fi, fo, n=20):
     return lat.timer_stop() / n
 
 
+def multi_rhs_run(L, formats=((0, 18), (2, 12)), rhs=(1, 4, 12, 24)):
+    """H on R right-hand sides sharing one gauge field: us per
+    application and per RHS, default launch shapes, 20 timed applications
+    after 3 untimed.  Returns {format: {R: {...}}}."""
+    from plqcd_amd import hopping as hp
+    V = int(np.prod(L))
+    u = hp.random_gauge_su3(V, 1)
+    lat = hp.Lattice(L)
+    out = {}
+    for prec, links in formats:
+        g = hp.Gauge(lat, prec, links).upload(u)
+        rec = {}
+        for R in rhs:
+            fi, fo = hp.Field(lat, prec, R), hp.Field(lat, prec, R)
+            for r in range(R):
+                fi.upload(hp.random_spinor(V, 10 + r), r)
+            ms = time_hop(hp, g, fi, fo)
+            rec[f"R{R}"] = {"us_per_application": round(ms * 1e3, 1), "us_per_rhs": round(ms * 1e3 / R, 1),
+                            "gflops": round(FLOP_PER_SITE * V * R / ms / 1e6, 1)}
+            fi.close(); fo.close()
+        out[f"{PREC_NAMES[prec]}_l{links}"] = rec
+        g.close()
+    lat.close()
+    return out
+
+
+def ragged_run(formats=((1, 12), (2, 12), (0, 18))):
+    """One launch of H over the ragged batch 16^3x32 + 24^3x48 + 32^3x64 with
+    its own gauge field and 12 RHS each against three separate
+    launches: medians of three alternating rounds of 20.  Returns {format: {...}}."""
+    import ctypes as C
+    from plqcd_amd import capi, hopping as hp
+    lib = capi.lib()
+    shapes = [(16, 16, 16, 32), (24, 24, 24, 48), (32, 32, 32, 64)]
+    sources = [[hp.random_spinor(int(np.prod(Ls)), 50 + r) for r in range(12)] for Ls in shapes]
+    gauges = [hp.random_gauge_su3(int(np.prod(Ls)), 5 + k) for k, Ls in enumerate(shapes)]
+    out = {}
+    for prec, links in formats:
+        lats, gs_, fis, fos = [], [], [], []
+        for k, Ls in enumerate(shapes):
+            la = hp.Lattice(Ls)
+            if lats:  # one stream for the whole batch
+                la.set_stream(lib.plqcd_hip_lattice_stream(lats[0].h))
+            lats.append(la)
+            gs_.append(hp.Gauge(la, prec, links).upload(gauges[k]))
+            fi, fo = hp.Field(la, prec, 12), hp.Field(la, prec, 12)
+            for r in range(12):
+                fi.upload(sources[k][r], r)
+            fis.append(fi); fos.append(fo)
+        n = len(shapes)
+        garr = (C.c_void_p * n)(*[x_.h.value for x_ in gs_])
+        iarr = (C.c_void_p * n)(*[x_.h.value for x_ in fis])
+        oarr = (C.c_void_p * n)(*[x_.h.value for x_ in fos])
+        for _ in range(3):
+            capi.check(lib.plqcd_hip_hop_batch(n, garr, iarr, oarr, 1), "hop_batch")
+        tb, ts = [], []
+        lat = lats[0]
+        for _ in range(3):
+            lat.timer_start()
+            for _ in range(20):
+                capi.check(lib.plqcd_hip_hop_batch(n, garr, iarr, oarr, 1), "hop_batch")
+            tb.append(lat.timer_stop() / 20)
+            lat.timer_start()
+            for _ in range(20):
+                for k in range(n):
+                    hp.hop(gs_[k], fis[k], fos[k], hp.PARITY_ALL, hp.HOP_WILSON)
+            ts.append(lat.timer_stop() / 20)
+        t_batch, t_sep = float(np.median(tb)), float(np.median(ts))
+        sites = sum(int(np.prod(Ls)) for Ls in shapes) * 12
+        out[f"{PREC_NAMES[prec]}_l{links}"] = {"us_one_launch": round(t_batch * 1e3, 1),
+                                               "us_separate_launches": round(t_sep * 1e3, 1),
+                                               "gflops_one_launch": round(FLOP_PER_SITE * sites / t_batch / 1e6, 1)}
+        for k in reversed(range(n)):
+            fis[k].close(); fos[k].close(); gs_[k].close(); lats[k].close()
+    return out
+
+
 def solver_suite():
     """configs[2] (48^3x96 CG: FP64 vs mixed), configs[3] (multi-RHS hop and
     CG at 32^3x64, ragged batch in one launch, autotuner).  One JSON line each."""
@@ -378,20 +461,9 @@ def solver_suite():
     # --- multi-RHS at 32^3 x 64
     L = (32, 32, 32, 64)
     V = int(np.prod(L))
-    u = hp.random_gauge_su3(V, 1)
-    lat = hp.Lattice(L)
-    for prec, links in ((0, 18), (1, 12), (2, 18), (2, 12)):
-        g = hp.Gauge(lat, prec, links).upload(u)
-        for R in (1, 4, 12, 24):
-            fi, fo = hp.Field(lat, prec, R), hp.Field(lat, prec, R)
-            for r in range(R):
-                fi.upload(hp.random_spinor(V, 10 + r), r)
-            ms = time_hop(hp, g, fi, fo)
-            emit({"suite": "solver", "case": f"hop_32c64_{PREC_NAMES[prec]}_l{links}_R{R}",
-                  "us_per_application": round(ms * 1e3, 1), "us_per_rhs": round(ms * 1e3 / R, 1),
-                  "gflops": round(FLOP_PER_SITE * V * R / ms / 1e6, 1)})
-            fi.close(); fo.close()
-        g.close()
+    for fmt, rec in multi_rhs_run(L, ((0, 18), (1, 12), (2, 18), (2, 12))).items():
+        for rk, r in rec.items():
+            emit({"suite": "solver", "case": f"hop_32c64_{fmt}_{rk}", **r})
     for R in (1, 4, 12, 24):
         for sl in ((1, 12), (2, 12)):
             rec = solve_compare(L, nrhs=R, sloppy=sl)
@@ -402,45 +474,10 @@ def solver_suite():
     rec.update({"suite": "solver", "case": "cg_32c64_mixed_f16_R12_vs_fp64_R1_solver_autotuned"})
     emit(rec)
     # ragged batch: 16^3x32, 24^3x48, 32^3x64, each 12 RHS, one launch
-    shapes = [(16, 16, 16, 32), (24, 24, 24, 48), (32, 32, 32, 64)]
-    sources = [[hp.random_spinor(int(np.prod(Ls)), 50 + r) for r in range(12)] for Ls in shapes]
-    for prec, links in ((1, 12), (2, 12), (0, 18)):
-        lats, gs_, fis, fos = [], [], [], []
-        for k, Ls in enumerate(shapes):
-            Vs = int(np.prod(Ls))
-            la = hp.Lattice(Ls)
-            la.set_stream(lib.plqcd_hip_lattice_stream(lat.h))
-            lats.append(la)
-            gs_.append(hp.Gauge(la, prec, links).upload(hp.random_gauge_su3(Vs, 5 + k)))
-            fi, fo = hp.Field(la, prec, 12), hp.Field(la, prec, 12)
-            for r in range(12):
-                fi.upload(sources[k][r], r)
-            fis.append(fi); fos.append(fo)
-        n = len(shapes)
-        garr = (C.c_void_p * n)(*[x_.h.value for x_ in gs_])
-        iarr = (C.c_void_p * n)(*[x_.h.value for x_ in fis])
-        oarr = (C.c_void_p * n)(*[x_.h.value for x_ in fos])
-        for _ in range(3):
-            capi.check(lib.plqcd_hip_hop_batch(n, garr, iarr, oarr, 1), "hop_batch")
-        # one launch and three launches alternate, three times; medians
-        tb, ts = [], []
-        for _ in range(3):
-            lat.timer_start()
-            for _ in range(20):
-                capi.check(lib.plqcd_hip_hop_batch(n, garr, iarr, oarr, 1), "hop_batch")
-            tb.append(lat.timer_stop() / 20)
-            lat.timer_start()
-            for _ in range(20):
-                for k in range(n):
-                    hp.hop(gs_[k], fis[k], fos[k], hp.PARITY_ALL, hp.HOP_WILSON)
-            ts.append(lat.timer_stop() / 20)
-        t_batch, t_sep = float(np.median(tb)), float(np.median(ts))
-        sites = sum(int(np.prod(Ls)) for Ls in shapes) * 12
-        emit({"suite": "solver", "case": f"ragged_batch_{PREC_NAMES[prec]}_l{links}_3lattices_12rhs",
-              "us_one_launch": round(t_batch * 1e3, 1), "us_separate_launches": round(t_sep * 1e3, 1),
-              "gflops_one_launch": round(FLOP_PER_SITE * sites / t_batch / 1e6, 1)})
-        for k in range(n):
-            fis[k].close(); fos[k].close(); gs_[k].close(); lats[k].close()
+    for fmt, r in ragged_run().items():
+        emit({"suite": "solver", "case": f"ragged_batch_{fmt}_3lattices_12rhs", **r})
+    u = hp.random_gauge_su3(V, 1)
+    lat = hp.Lattice(L)
     # autotuner at 32^3 x 64
     for prec, links, R in ((0, 18, 1), (1, 12, 12), (2, 12, 12)):
         g = hp.Gauge(lat, prec, links).upload(u)
@@ -541,7 +578,7 @@ def calib_suite():
 def serve_suite(duration: float, rate: float, seed: int = 7, pinned: bool = False):
     """Poisson request stream for `duration` seconds; volumes from
     {16^4, 24^3x48, 32^3x64, 48^3x96}, 1..12 RHS, tol in {1e-8,1e-10,1e-12},
-    kappa in [0.120, 0.1245], two gauge fields per volume.  Reports p50/p99
+    kappa in [0.120, 0.1245], two gauge fields per volume.  Returns p50/p99
     latency and solves/s beside one-at-a-time plain FP64 solves.  With `pinned`
     the sources live in page-locked host memory (moved by DMA alone)."""
     from plqcd_amd import hopping as hp, solver as sv
@@ -608,7 +645,7 @@ def serve_suite(duration: float, rate: float, seed: int = 7, pinned: bool = Fals
         x.close(); b.close(); g.close(); la.close()
     base_seconds = sum(per_rhs[submitted[i]["L"]] * submitted[i]["nrhs"] for i in done)
     base_io_seconds = sum(per_rhs_io[submitted[i]["L"]] * submitted[i]["nrhs"] for i in done)
-    emit({"suite": "serve", "case": "poisson_mixed_volumes", "duration_s": duration, "rate_per_s": rate,
+    return {"suite": "serve", "case": "poisson_mixed_volumes", "duration_s": duration, "rate_per_s": rate,
           "pinned_sources": pinned,
           "requests": len(done), "rhs_solved": nrhs, "all_converged": ok, "wall_s": round(wall, 2),
           "p50_latency_s": round(float(np.percentile(lat_s, 50)), 4),
@@ -621,7 +658,7 @@ def serve_suite(duration: float, rate: float, seed: int = 7, pinned: bool = Fals
           "hbm_budget_gb": round(info["budget"] / 2**30, 1), "failed": info["failed"],
           "engines_created": info["engines_created"], "evictions": info["evictions"],
           "host_transfer_gb": round(info["transfer_bytes"] / 1e9, 1),
-          "host_transfer_thread_seconds": round(info["transfer_seconds"], 1)})
+          "host_transfer_thread_seconds": round(info["transfer_seconds"], 1)}
 
 
 def main():
@@ -637,6 +674,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solver", action="store_true")
     ap.add_argument("--no-formats", action="store_true", help="skip the FP32/FP16 storage-format timings")
+    ap.add_argument("--serve-seconds", type=float, default=10.0,
+                    help="hop line: seconds of the short Poisson serving run (0: none)")
     ap.add_argument("--suite", choices=["hop", "solver", "serve", "calib", "cg"], default="hop",
                     help="hop: the headline line (default); solver/serve: the extended studies, one JSON line "
                          "each; cg: the bench line's solver comparison alone")
@@ -653,7 +692,7 @@ def main():
         solver_suite()
         return
     if args.suite == "serve":
-        serve_suite(args.duration, args.rate, pinned=args.pinned)
+        emit(serve_suite(args.duration, args.rate, pinned=args.pinned))
         return
     if args.suite == "calib":
         calib_suite()
@@ -684,6 +723,9 @@ def main():
     dropin = dropin_run(L) if headline and not args.no_formats else None
     solver = None if args.no_solver else solve_compare(L)
     cg48 = cg48_compare() if headline and not args.no_solver else None
+    multi = multi_rhs_run(L) if headline and not args.no_solver else None
+    ragged = ragged_run(((1, 12), (0, 18))) if headline and not args.no_solver else None
+    serve = serve_suite(args.serve_seconds, 2.0) if headline and args.serve_seconds > 0 else None
     ceiling = mix_ceiling(L) if headline else None
     formats = None if (args.no_formats or prec != 0) else formats_run(
         L, mode, headline_case=case if (prec, links) == (0, 18) else None)
@@ -727,6 +769,9 @@ def main():
         "dropin": dropin,
         "solver": solver,
         "cg_48c96": cg48,
+        "multi_rhs": multi,
+        "ragged_batch": ragged,
+        "serve": serve,
     }
     print(json.dumps(rec), flush=True)
 

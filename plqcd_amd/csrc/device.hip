@@ -491,7 +491,7 @@ int plqcd::Stager::upload(plqcd_hip_field *f, int rhs, const spinor *host)
    for (int i0 = 0, c = 0; i0 < V; i0 += per, c++) {
       const int b = c & 1, n = std::min(per, V - i0);
       if (used[b]) PLQCD_HIP_TRY(hipEventSynchronize(ev_[b]));  // buffer b's previous chunk is packed
-      plqcd::host_copy(pin_[b], host + i0, (size_t)n * sizeof(spinor));
+      std::memcpy(pin_[b], host + i0, (size_t)n * sizeof(spinor));
       PLQCD_HIP_TRY(hipMemcpyAsync(dev_[b], pin_[b], (size_t)n * sizeof(spinor), hipMemcpyHostToDevice, st_));
       const d2 *src = static_cast<const d2 *>(dev_[b]);
       const dim3 grid((n + kPackBlock - 1) / kPackBlock), block(kPackBlock);
@@ -527,7 +527,7 @@ int plqcd::Stager::download(const plqcd_hip_field *f, int rhs, spinor *host)
       PLQCD_HIP_TRY(hipEventRecord(ev_[b], st_));
       if (pend_i0 >= 0) {  // copy out the previous chunk while this one is in flight
          PLQCD_HIP_TRY(hipEventSynchronize(ev_[pend_b]));
-         plqcd::host_copy(host + pend_i0, pin_[pend_b], (size_t)pend_n * sizeof(spinor));
+         std::memcpy(host + pend_i0, pin_[pend_b], (size_t)pend_n * sizeof(spinor));
       }
       pend_i0 = i0;
       pend_n = n;
@@ -535,7 +535,7 @@ int plqcd::Stager::download(const plqcd_hip_field *f, int rhs, spinor *host)
    }
    if (pend_i0 >= 0) {
       PLQCD_HIP_TRY(hipEventSynchronize(ev_[pend_b]));
-      plqcd::host_copy(host + pend_i0, pin_[pend_b], (size_t)pend_n * sizeof(spinor));
+      std::memcpy(host + pend_i0, pin_[pend_b], (size_t)pend_n * sizeof(spinor));
    }
    return 0;
 }
@@ -598,7 +598,7 @@ int plqcd::Stager::upload_raw(void *dev, const void *host, size_t bytes)
       const int b = (int)(c & 1);
       const size_t n = std::min(kStagerChunk, bytes - o);
       if (used[b]) PLQCD_HIP_TRY(hipEventSynchronize(ev_[b]));  // buffer b's previous chunk has left
-      plqcd::host_copy(pin_[b], static_cast<const char *>(host) + o, n);
+      std::memcpy(pin_[b], static_cast<const char *>(host) + o, n);
       PLQCD_HIP_TRY(hipMemcpyAsync(static_cast<char *>(dev) + o, pin_[b], n, hipMemcpyHostToDevice, st_));
       PLQCD_HIP_TRY(hipEventRecord(ev_[b], st_));
       used[b] = true;
@@ -629,7 +629,7 @@ int plqcd::Stager::download_raw(void *host, const void *dev, size_t bytes, hipEv
       PLQCD_HIP_TRY(hipEventRecord(ev_[b], st_));
       if (pend_b >= 0) {  // copy out the previous chunk while this one is in flight
          PLQCD_HIP_TRY(hipEventSynchronize(ev_[pend_b]));
-         plqcd::host_copy(static_cast<char *>(host) + pend_o, pin_[pend_b], pend_n);
+         std::memcpy(static_cast<char *>(host) + pend_o, pin_[pend_b], pend_n);
       }
       pend_o = o;
       pend_n = n;
@@ -637,7 +637,7 @@ int plqcd::Stager::download_raw(void *host, const void *dev, size_t bytes, hipEv
    }
    if (pend_b >= 0) {
       PLQCD_HIP_TRY(hipEventSynchronize(ev_[pend_b]));
-      plqcd::host_copy(static_cast<char *>(host) + pend_o, pin_[pend_b], pend_n);
+      std::memcpy(static_cast<char *>(host) + pend_o, pin_[pend_b], pend_n);
    }
    return 0;
 }

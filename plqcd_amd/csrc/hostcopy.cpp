@@ -4,7 +4,9 @@
 // that, so a transfer that copies on one thread is bound by the copy.  A small
 // pool of workers, started on first use and kept for the life of the process,
 // splits each large copy into page-aligned pieces; the calling thread takes a
-// piece too and returns when every piece is done.
+// piece too and returns when every piece is done.  Copies from several
+// threads take turns, so paths that already copy on many threads at once
+// (the server's stagers) keep their own memcpy.
 #include <sched.h>
 
 #include <algorithm>
