@@ -8,6 +8,7 @@
 // threads take turns, so paths that already copy on many threads at once
 // (the server's stagers) keep their own memcpy.
 #include <sched.h>
+#include <unistd.h>
 
 #include <algorithm>
 #include <condition_variable>
@@ -37,16 +38,21 @@ int pool_threads()
 
 class CopyPool {
   public:
+   // never destroyed: the workers stay blocked on the pool's condition
+   // variable until the process ends (joining them from a static destructor
+   // would run while other threads may still copy, and a forked child holds
+   // thread objects of threads it does not have)
    static CopyPool &get()
    {
-      static CopyPool pool;
-      return pool;
+      static CopyPool *pool = new CopyPool();
+      return *pool;
    }
 
    void copy(void *dst, const void *src, size_t n)
    {
       const int parts = nworkers_ + 1;
-      if (n < kMinParallel || parts == 1) {
+      // a child forked after the pool started has none of its threads
+      if (n < kMinParallel || parts == 1 || getpid() != owner_) {
          std::memcpy(dst, src, n);
          return;
       }
@@ -71,19 +77,10 @@ class CopyPool {
   private:
    CopyPool()
    {
+      owner_ = getpid();
       nworkers_ = pool_threads() - 1;
       for (int k = 0; k < nworkers_; k++) th_.emplace_back([this, k] { worker(k + 1); });
    }
-   ~CopyPool()
-   {
-      {
-         std::lock_guard<std::mutex> lk(mu_);
-         stop_ = true;
-      }
-      cv_.notify_all();
-      for (std::thread &t : th_) t.join();
-   }
-
    void run_piece(int k)
    {
       const size_t o = size_t(k) * piece_;
@@ -97,8 +94,7 @@ class CopyPool {
       for (;;) {
          {
             std::unique_lock<std::mutex> lk(mu_);
-            cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
-            if (stop_) return;
+            cv_.wait(lk, [&] { return gen_ != seen; });
             seen = gen_;
          }
          run_piece(k);
@@ -109,13 +105,13 @@ class CopyPool {
       }
    }
 
+   pid_t owner_ = 0;
    int nworkers_ = 0;
    std::vector<std::thread> th_;
    std::mutex call_mu_, mu_;
    std::condition_variable cv_, done_cv_;
    uint64_t gen_ = 0;
    int pending_ = 0;
-   bool stop_ = false;
    char *dst_ = nullptr;
    const char *src_ = nullptr;
    size_t n_ = 0, piece_ = 0;
