@@ -163,7 +163,9 @@ typedef struct plqcd_hip_field plqcd_hip_field;
 /* A lattice with extents L[0..3] (x,y,z,t), each even and >= 2, with at most
  * 2^29 sites (the kernels' 32-bit link offsets), on the
  * current HIP device, with its own stream.  All calls taking a lattice
- * object are asynchronous on that stream unless stated. */
+ * object are asynchronous on that stream unless stated.  Host transfers of
+ * a lattice's fields and links share its staging buffers: one thread at a
+ * time transfers to or from the fields of one lattice. */
 int plqcd_hip_lattice_create(const int L[4], plqcd_hip_lattice **out);
 void plqcd_hip_lattice_destroy(plqcd_hip_lattice *lat);
 /* Use an existing hipStream_t (NULL: back to the lattice's own stream). */

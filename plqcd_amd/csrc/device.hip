@@ -155,9 +155,8 @@ int ensure_xfer(plqcd_hip_lattice *lat)
 // pack(device chunk, first element, elements) enqueues the repacking kernel
 // on the lattice stream.  Returns when the last chunk has been repacked.
 template <class Pack>
-int upload_chunks(plqcd_hip_lattice *lat, const void *host, size_t esize, size_t first, size_t count, Pack pack)
+int upload_chunks_(plqcd_hip_lattice *lat, const void *host, size_t esize, size_t first, size_t count, Pack pack)
 {
-   if (int rc = ensure_xfer(lat)) return rc;
    hipStream_t st = lat->stream;
    const size_t per = kXferBytes / esize;
    const char *h = static_cast<const char *>(host);
@@ -186,9 +185,8 @@ int upload_chunks(plqcd_hip_lattice *lat, const void *host, size_t esize, size_t
 // first element, elements) enqueues the kernel that writes a chunk in the host
 // layout; each chunk's DMA overlaps the copy-out of the previous one.
 template <class Unpack>
-int download_chunks(plqcd_hip_lattice *lat, void *host, size_t esize, size_t first, size_t count, Unpack unpack)
+int download_chunks_(plqcd_hip_lattice *lat, void *host, size_t esize, size_t first, size_t count, Unpack unpack)
 {
-   if (int rc = ensure_xfer(lat)) return rc;
    hipStream_t st = lat->stream;
    const size_t per = kXferBytes / esize;
    char *h = static_cast<char *>(host);
@@ -218,6 +216,25 @@ int download_chunks(plqcd_hip_lattice *lat, void *host, size_t esize, size_t fir
    }
    PLQCD_HIP_TRY(hipStreamSynchronize(st));
    return 0;
+}
+
+// A transfer that fails part-way leaves no DMA in flight from the staging
+// buffers, which the next transfer of the lattice reuses.
+template <class Pack>
+int upload_chunks(plqcd_hip_lattice *lat, const void *host, size_t esize, size_t first, size_t count, Pack pack)
+{
+   if (int rc = ensure_xfer(lat)) return rc;
+   const int rc = upload_chunks_(lat, host, esize, first, count, pack);
+   if (rc) (void)hipStreamSynchronize(lat->stream);
+   return rc;
+}
+template <class Unpack>
+int download_chunks(plqcd_hip_lattice *lat, void *host, size_t esize, size_t first, size_t count, Unpack unpack)
+{
+   if (int rc = ensure_xfer(lat)) return rc;
+   const int rc = download_chunks_(lat, host, esize, first, count, unpack);
+   if (rc) (void)hipStreamSynchronize(lat->stream);
+   return rc;
 }
 
 int check_rhs(const plqcd_hip_field *f, int rhs)
