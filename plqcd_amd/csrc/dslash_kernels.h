@@ -862,7 +862,10 @@ constexpr size_t tile_lds_bytes()
 // barrier; every lane of the block calls this.  In compat mode the forward
 // link is the even site's with the same index j (slot sx in the tile): with
 // XSH an odd lane reads it from ux after the barrier, otherwise from memory.
-template <int MU, int MODE, bool DAG, int BLOCK, bool XSH, class SIO, class GIO>
+// NB: where the neighbours are -- 0 always in the tile (x: whole rows), 2
+// never (z with one-plane tiles), 1 either, by lane (slot >= 0: in the tile).
+// With NB 1 the compiler merges the two reads into one flat load.
+template <int MU, int NB, int MODE, bool DAG, int BLOCK, bool XSH, class SIO, class GIO>
 __device__ __forceinline__ void tile_dir(const SIO &in, const GIO &g, const typename SIO::Sp::E (*buf)[BLOCK], int p,
                                          int j, int base, int cu, int cd, int su, int sd, int sx,
                                          typename SIO::C acc[12], typename GIO::U::E (*ux)[BLOCK])
@@ -874,12 +877,15 @@ __device__ __forceinline__ void tile_dir(const SIO &in, const GIO &g, const type
    Sp f;
    typename GIO::U u;
    typename SIO::Hv r;
-   if (su >= 0) {
+   auto nb_load = [&](int slot, int jnb) {
+      if (NB == 0 || (NB == 1 && slot >= 0)) {
 #pragma unroll
-      for (int m = 0; m < Sp::N; m++) f.e[m] = buf[m][su];
-   } else {
-      in.load_sp(1 - p, cu + base, f);
-   }
+         for (int m = 0; m < Sp::N; m++) f.e[m] = buf[m][slot];
+      } else {
+         in.load_sp(1 - p, jnb + base, f);
+      }
+   };
+   nb_load(su, cu);
    if constexpr (share) {
       g.template load_u<true>(p, MU, j, u);
 #pragma unroll
@@ -896,12 +902,7 @@ __device__ __forceinline__ void tile_dir(const SIO &in, const GIO &g, const type
    }
    term_u<MU, SG>(f, u, r);
    accum<MU, SG, !compat>(acc, r);
-   if (sd >= 0) {
-#pragma unroll
-      for (int m = 0; m < Sp::N; m++) f.e[m] = buf[m][sd];
-   } else {
-      in.load_sp(1 - p, cd + base, f);
-   }
+   nb_load(sd, cd);
    if constexpr (share) {
       if constexpr (!compat) __syncthreads();
 #pragma unroll
@@ -943,6 +944,7 @@ __global__ __launch_bounds__(BLOCK) void hop_tile_kernel(SIO in, SIO out, GIO g,
    // 18-real, 32x4x1 tiles at 32^3 x 64: 407 against 427 us).  FP32 tiles ran
    // up to 27 % slower with it (12-real, 32x4x1, 16 slices: 220 against 173 us).
    constexpr bool XSH = tile_xsh<SIO, GIO, BLOCK>();
+   constexpr bool SPLIT = std::is_same<typename SIO::real, double>::value;
    __shared__ typename GIO::U::E ux[XSH ? 9 : 1][XSH ? BLOCK : 1];  // the lanes' forward x links of this slice
    typedef typename GIO::U::E(*UxRow)[BLOCK];
    const int S = a.S;
@@ -1019,11 +1021,31 @@ __global__ __launch_bounds__(BLOCK) void hop_tile_kernel(SIO in, SIO out, GIO g,
       for (int m = 0; m < 12; m++) acc[m] = C{0, 0};
       // the scheduling barriers keep each direction's loads in its own
       // section, which holds the kernel within 256 VGPRs without spills
-      tile_dir<0, MODE, DAG, BLOCK, XSH>(in, g, buf, p, j, base, cu[0], cd[0], su[0], sd[0], sx, acc, (UxRow)ux);
-      __builtin_amdgcn_sched_barrier(0);
-      tile_dir<1, MODE, DAG, BLOCK, XSH>(in, g, buf, p, j, base, cu[1], cd[1], su[1], sd[1], sx, acc, (UxRow)ux);
-      __builtin_amdgcn_sched_barrier(0);
-      tile_dir<2, MODE, DAG, BLOCK, XSH>(in, g, buf, p, j, base, cu[2], cd[2], su[2], sd[2], sx, acc, (UxRow)ux);
+      // FP64: the x neighbours are read from LDS with no test, and the z
+      // neighbours of one-plane tiles from memory with none (each neighbour
+      // read under a per-lane test became one flat load serving both
+      // places, waited for with every global load in flight; FP64 12-real
+      // 32x1x4 tiles at 32^3 x 64: 333 against 351 us, 18-real unchanged).
+      // FP32 keeps the per-lane test in every direction (18-real 32x8x1
+      // tiles ran 2.5 % slower split).
+      if constexpr (SPLIT) {
+         tile_dir<0, 0, MODE, DAG, BLOCK, XSH>(in, g, buf, p, j, base, cu[0], cd[0], su[0], sd[0], sx, acc, (UxRow)ux);
+         __builtin_amdgcn_sched_barrier(0);
+         tile_dir<1, 1, MODE, DAG, BLOCK, XSH>(in, g, buf, p, j, base, cu[1], cd[1], su[1], sd[1], sx, acc, (UxRow)ux);
+         __builtin_amdgcn_sched_barrier(0);
+         if (a.tz == 1)
+            tile_dir<2, 2, MODE, DAG, BLOCK, XSH>(in, g, buf, p, j, base, cu[2], cd[2], su[2], sd[2], sx, acc,
+                                                 (UxRow)ux);
+         else
+            tile_dir<2, 1, MODE, DAG, BLOCK, XSH>(in, g, buf, p, j, base, cu[2], cd[2], su[2], sd[2], sx, acc,
+                                                 (UxRow)ux);
+      } else {
+         tile_dir<0, 1, MODE, DAG, BLOCK, XSH>(in, g, buf, p, j, base, cu[0], cd[0], su[0], sd[0], sx, acc, (UxRow)ux);
+         __builtin_amdgcn_sched_barrier(0);
+         tile_dir<1, 1, MODE, DAG, BLOCK, XSH>(in, g, buf, p, j, base, cu[1], cd[1], su[1], sd[1], sx, acc, (UxRow)ux);
+         __builtin_amdgcn_sched_barrier(0);
+         tile_dir<2, 1, MODE, DAG, BLOCK, XSH>(in, g, buf, p, j, base, cu[2], cd[2], su[2], sd[2], sx, acc, (UxRow)ux);
+      }
       __builtin_amdgcn_sched_barrier(0);
       {
          Sp f;
