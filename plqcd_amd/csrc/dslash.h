@@ -35,9 +35,9 @@ extern HopTuning g_tuning[3];  // per storage precision
 namespace plqcd {
 // Launch shape for `R` right-hand sides of storage format (prec, links) before
 // the per-shape cache: the precision's shape set by plqcd_hip_hop_tuning, else
-// a built-in default for R = 1 (per format) or for R > 1 (per precision, and
-// whether the launch applies M = 1 - kappa H with or without the CG update:
-// `solver`).
+// a built-in default for R = 1 (per format) or for R > 1 (per precision), and
+// in both cases by whether the launch applies M = 1 - kappa H with or without
+// the CG update (`solver`) where that differs.
 HopTuning default_tuning(int prec, int links, int R, bool solver = false);
 }  // namespace plqcd
 

@@ -8,9 +8,11 @@
 #include "dslash.h"
 
 // Built-in launch shapes, from sweeps at 32^3 x 64.  One RHS, per storage
-// format: the LDS-tiled t-march for F64 (tiles of 32 x 4 x 1 sites, 32
-// t-slices: 434 us against 514 us for the untiled t-march; 32 x 1 x 4 with
-// 12-real links: 351 against 356 us for 32 x 4 x 1) and F32 (32 x 8 x 1 sites
+// format: the LDS-tiled t-march for F64 (tiles of 32 x 1 x 4 sites, 32
+// t-slices: 400 us with 18-real links against 414 us for 32 x 4 x 1 since
+// the x and one-plane z neighbour reads lost their per-lane test, and 434 us
+// for 32 x 4 x 1 and 514 us untiled before that; 12-real links: 331 us) and
+// F32 (32 x 8 x 1 sites
 // and 32 t-slices with 18-real links: 247 against 264 us for 32 x 4 x 2 and 8
 // t-slices; 32 x 4 x 4 and 8 t-slices with 12-real links: 171 us), the
 // untiled t-march over 8 t-slices in 512-lane blocks for F16 (102 us with
@@ -26,12 +28,17 @@ struct FormatShape {
    int prec, links;
    HopTuning t;
 };
-const FormatShape kOneRhs[6] = {{PLQCD_PREC_F64, 18, {1, 1, 256, 32, 1, 4, 1}},
+const FormatShape kOneRhs[6] = {{PLQCD_PREC_F64, 18, {1, 1, 256, 32, 1, 1, 4}},
                                 {PLQCD_PREC_F64, 12, {1, 1, 256, 32, 1, 1, 4}},
                                 {PLQCD_PREC_F32, 18, {1, 1, 256, 32, 1, 8, 1}},
                                 {PLQCD_PREC_F32, 12, {2, 8, 256, 8, 1, 4, 4}},
                                 {PLQCD_PREC_F16, 12, {1, 1, 512, 8, 1, 0, 0}},
                                 {PLQCD_PREC_F16, 18, {1, 1, 512, 8, 1, 0, 0}}};
+// One RHS with the solver epilogues (M with |Mp|^2, M^+ with the CG update),
+// where they differ from plain H: F64 18-real keeps 32 x 4 x 1 tiles (one CG
+// iteration's pair of launches 1.149 ms at 32^3 x 64; with 32 x 1 x 4 the
+// FP64 one-RHS solve took 0.0806 against 0.0769 s).
+const FormatShape kOneRhsSolver[1] = {{PLQCD_PREC_F64, 18, {1, 1, 256, 32, 1, 4, 1}}};
 const HopTuning kMultiRhs[3] = {{2, 8, 256, 16, 1, 0, 0}, {2, 8, 256, 8, 1, 0, 0}, {2, 8, 256, 8, 1, 0, 0}};
 // Several RHS with the solver epilogues (M, and M^+ with the CG update): the
 // F16 gather kernel, whose 64-72 VGPRs leave room for more waves to overlap
@@ -58,6 +65,9 @@ HopTuning g_tuning[3] = {one_rhs_shape(PLQCD_PREC_F64, 18), one_rhs_shape(PLQCD_
 HopTuning plqcd::default_tuning(int prec, int links, int R, bool solver)
 {
    if (g_tuning_set[prec]) return g_tuning[prec];
+   if (R <= 1 && solver)
+      for (const FormatShape &f : kOneRhsSolver)
+         if (f.prec == prec && f.links == links) return f.t;
    if (R <= 1) return one_rhs_shape(prec, links);
    return solver ? kMultiRhsSolver[prec] : kMultiRhs[prec];
 }
