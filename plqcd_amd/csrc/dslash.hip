@@ -14,7 +14,8 @@
 // for 32 x 4 x 1 and 514 us untiled before that; 12-real links: 331 us) and
 // F32 (32 x 8 x 1 sites
 // and 32 t-slices with 18-real links: 247 against 264 us for 32 x 4 x 2 and 8
-// t-slices; 32 x 4 x 4 and 8 t-slices with 12-real links: 171 us), the
+// t-slices; 32 x 8 x 2 and 8 t-slices with 12-real links: 168 against 176 us
+// for 32 x 4 x 4), the
 // untiled t-march over 8 t-slices in 512-lane blocks for F16 (102 us with
 // 12-real links, the gather kernel 116 us; 118 against 124 us with 18-real
 // links for 4 t-slices).  Shapes that do not
@@ -31,7 +32,7 @@ struct FormatShape {
 const FormatShape kOneRhs[6] = {{PLQCD_PREC_F64, 18, {1, 1, 256, 32, 1, 1, 4}},
                                 {PLQCD_PREC_F64, 12, {1, 1, 256, 32, 1, 1, 4}},
                                 {PLQCD_PREC_F32, 18, {1, 1, 256, 32, 1, 8, 1}},
-                                {PLQCD_PREC_F32, 12, {2, 8, 256, 8, 1, 4, 4}},
+                                {PLQCD_PREC_F32, 12, {1, 1, 256, 8, 1, 8, 2}},
                                 {PLQCD_PREC_F16, 12, {1, 1, 512, 8, 1, 0, 0}},
                                 {PLQCD_PREC_F16, 18, {1, 1, 512, 8, 1, 0, 0}}};
 // One RHS with the solver epilogues (M with |Mp|^2, M^+ with the CG update),
