@@ -1021,31 +1021,21 @@ __global__ __launch_bounds__(BLOCK) void hop_tile_kernel(SIO in, SIO out, GIO g,
       for (int m = 0; m < 12; m++) acc[m] = C{0, 0};
       // the scheduling barriers keep each direction's loads in its own
       // section, which holds the kernel within 256 VGPRs without spills
-      // FP64: the x neighbours are read from LDS with no test, and the z
-      // neighbours of one-plane tiles from memory with none (each neighbour
-      // read under a per-lane test became one flat load serving both
-      // places, waited for with every global load in flight; FP64 12-real
-      // 32x1x4 tiles at 32^3 x 64: 333 against 351 us, 18-real unchanged).
-      // FP32 keeps the per-lane test in every direction (18-real 32x8x1
-      // tiles ran 2.5 % slower split).
-      if constexpr (SPLIT) {
-         tile_dir<0, 0, MODE, DAG, BLOCK, XSH>(in, g, buf, p, j, base, cu[0], cd[0], su[0], sd[0], sx, acc, (UxRow)ux);
-         __builtin_amdgcn_sched_barrier(0);
-         tile_dir<1, 1, MODE, DAG, BLOCK, XSH>(in, g, buf, p, j, base, cu[1], cd[1], su[1], sd[1], sx, acc, (UxRow)ux);
-         __builtin_amdgcn_sched_barrier(0);
-         if (a.tz == 1)
-            tile_dir<2, 2, MODE, DAG, BLOCK, XSH>(in, g, buf, p, j, base, cu[2], cd[2], su[2], sd[2], sx, acc,
-                                                 (UxRow)ux);
-         else
-            tile_dir<2, 1, MODE, DAG, BLOCK, XSH>(in, g, buf, p, j, base, cu[2], cd[2], su[2], sd[2], sx, acc,
-                                                 (UxRow)ux);
-      } else {
-         tile_dir<0, 1, MODE, DAG, BLOCK, XSH>(in, g, buf, p, j, base, cu[0], cd[0], su[0], sd[0], sx, acc, (UxRow)ux);
-         __builtin_amdgcn_sched_barrier(0);
-         tile_dir<1, 1, MODE, DAG, BLOCK, XSH>(in, g, buf, p, j, base, cu[1], cd[1], su[1], sd[1], sx, acc, (UxRow)ux);
-         __builtin_amdgcn_sched_barrier(0);
+      // The x neighbours (whole rows) are read from LDS with no test, and in
+      // FP64 the z neighbours of one-plane tiles from memory with none: each
+      // neighbour read under a per-lane test became one flat load serving
+      // both places, waited for with every global load in flight (FP64
+      // 12-real 32x1x4 tiles at 32^3 x 64: 333 against 351 us; FP32 18-real
+      // 32x8x1: 240 against 245 us with the x test gone, 2.5 % slower with
+      // the z one gone too, so FP32 keeps that).
+      tile_dir<0, 0, MODE, DAG, BLOCK, XSH>(in, g, buf, p, j, base, cu[0], cd[0], su[0], sd[0], sx, acc, (UxRow)ux);
+      __builtin_amdgcn_sched_barrier(0);
+      tile_dir<1, 1, MODE, DAG, BLOCK, XSH>(in, g, buf, p, j, base, cu[1], cd[1], su[1], sd[1], sx, acc, (UxRow)ux);
+      __builtin_amdgcn_sched_barrier(0);
+      if (SPLIT && a.tz == 1)
+         tile_dir<2, 2, MODE, DAG, BLOCK, XSH>(in, g, buf, p, j, base, cu[2], cd[2], su[2], sd[2], sx, acc, (UxRow)ux);
+      else
          tile_dir<2, 1, MODE, DAG, BLOCK, XSH>(in, g, buf, p, j, base, cu[2], cd[2], su[2], sd[2], sx, acc, (UxRow)ux);
-      }
       __builtin_amdgcn_sched_barrier(0);
       {
          Sp f;
