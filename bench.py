@@ -344,9 +344,11 @@ def emit(rec):
     print(json.dumps(rec), flush=True)
 
 
-def time_hop(hp, g, fi, fo, n=20):
+def time_hop(hp, g, fi, fo, n=20, warmup=10):
+    """us per application of H over n launches after `warmup` untimed ones
+    (fresh fields and a GPU coming out of other work need a few)."""
     lat = g.lat
-    for _ in range(3):
+    for _ in range(warmup):
         hp.hop(g, fi, fo, hp.PARITY_ALL, hp.HOP_WILSON)
     lat.timer_start()
     for _ in range(n):
@@ -357,7 +359,7 @@ def time_hop(hp, g, fi, fo, n=20):
 def multi_rhs_run(L, formats=((0, 18), (2, 12)), rhs=(1, 4, 12, 24)):
     """H on R right-hand sides sharing one gauge field: us per
     application and per RHS, default launch shapes, 20 timed applications
-    after 3 untimed.  Returns {format: {R: {...}}}."""
+    after 10 untimed.  Returns {format: {R: {...}}}."""
     from plqcd_amd import hopping as hp
     V = int(np.prod(L))
     u = hp.random_gauge_su3(V, 1)
