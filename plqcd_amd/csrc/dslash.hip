@@ -12,13 +12,13 @@
 // t-slices: 400 us with 18-real links against 414 us for 32 x 4 x 1 since
 // the x and one-plane z neighbour reads lost their per-lane test, and 434 us
 // for 32 x 4 x 1 and 514 us untiled before that; 12-real links: 331 us) and
-// F32 (32 x 8 x 1 sites
-// and 32 t-slices with 18-real links: 247 against 264 us for 32 x 4 x 2 and 8
-// t-slices; 32 x 8 x 2 and 8 t-slices with 12-real links: 168 against 176 us
-// for 32 x 4 x 4), the
-// untiled t-march over 8 t-slices in 512-lane blocks for F16 (102 us with
-// 12-real links, the gather kernel 116 us; 118 against 124 us with 18-real
-// links for 4 t-slices).  Shapes that do not
+// F32 (32 x 1 x 4 sites
+// and 32 t-slices with 18-real links: 237 against 240 us for 32 x 8 x 1; 32 x
+// 8 x 2 and 8 t-slices with 12-real links: 168 against 176 us for 32 x 4 x
+// 4), for F16 with 18-real links 32 x 8 x 1 tiles over 4 t-slices in XCD runs
+// of 16 tiles (115 against 117 us for the untiled t-march), and with 12-real
+// links the untiled t-march over 8 t-slices in 512-lane blocks (90-102 us,
+// the gather kernel 116 us).  Shapes that do not
 // fit a lattice fall back to the untiled t-march or the gather kernel.  Several
 // RHS per launch, per precision: XCDs sweeping the lattice together, with
 // t-marches of 8 slices for F32/F16 (F16 at 12 RHS: 80 us per RHS, the
@@ -31,10 +31,10 @@ struct FormatShape {
 };
 const FormatShape kOneRhs[6] = {{PLQCD_PREC_F64, 18, {1, 1, 256, 32, 1, 1, 4}},
                                 {PLQCD_PREC_F64, 12, {1, 1, 256, 32, 1, 1, 4}},
-                                {PLQCD_PREC_F32, 18, {1, 1, 256, 32, 1, 8, 1}},
+                                {PLQCD_PREC_F32, 18, {1, 1, 256, 32, 1, 1, 4}},
                                 {PLQCD_PREC_F32, 12, {1, 1, 256, 8, 1, 8, 2}},
                                 {PLQCD_PREC_F16, 12, {1, 1, 512, 8, 1, 0, 0}},
-                                {PLQCD_PREC_F16, 18, {1, 1, 512, 8, 1, 0, 0}}};
+                                {PLQCD_PREC_F16, 18, {2, 16, 256, 4, 1, 8, 1}}};
 // One RHS with the solver epilogues (M with |Mp|^2, M^+ with the CG update),
 // where they differ from plain H: F64 18-real keeps 32 x 4 x 1 tiles (one CG
 // iteration's pair of launches 1.149 ms at 32^3 x 64; with 32 x 1 x 4 the
