@@ -95,9 +95,9 @@ class CgEngine {
    plqcd_hip_field *rs_ = nullptr, *ps_ = nullptr, *qs_ = nullptr, *xs_ = nullptr;  // sloppy
    double *partial_ = nullptr;
    double *scal_ = nullptr;  // kappa, rr, rr_new, pap, alpha, beta, tmp (7 x S)
-   int *imask_ = nullptr;    // active, update (2 x S)
+   int *imask_ = nullptr;    // active, update, the list of slots being updated (3 x S)
    double *host_ = nullptr;  // pinned: rr_new, tmp (2 x S)
-   int *hmask_ = nullptr;    // pinned: active, update (2 x S), the sources of their uploads
+   int *hmask_ = nullptr;    // pinned: active, update, update list (3 x S), the sources of their uploads
    hipEvent_t ev_ = nullptr; // recorded after each enqueued iteration / reliable update
    struct Graph {
       hipGraph_t def = nullptr;
@@ -116,6 +116,7 @@ class CgEngine {
    double *tmp_d() { return scal_ + 6 * S_; }
    int *active_d() { return imask_; }
    int *update_d() { return imask_ + S_; }
+   int *ulist_d() { return imask_ + 2 * S_; }
 };
 
 }  // namespace plqcd

@@ -16,6 +16,7 @@ struct HopArgs {
    int T;        // t-slices per thread (t-march)
    int R;        // right-hand sides per launch: lanes per site
    int r0;       // RHS of the fields that lane 0 of a site handles (the launch covers r0 .. r0+R-1)
+   const int *rlist;     // when set: the launch covers RHS rlist[0 .. R-1] instead (device array)
    int spb;      // sites per block
    int ty, tz;   // tile extents in y and z (LDS-tiled kernel)
    int nchunk;   // reduction chunks per parity
@@ -58,8 +59,11 @@ namespace plqcd {
 //                    to partial
 //   nr      > 0:     only the right-hand sides r0 .. r0+nr-1 of the fields (kappa,
 //                    alpha and partial stay indexed by the field's RHS)
+//   rlist   != null: only the nr right-hand sides rlist[0 .. nr-1] (a device
+//                    array of distinct RHS of the fields; r0 is not used)
 struct DslashOp {
    int r0 = 0, nr = 0;
+   const int *rlist = nullptr;
    int dag = 0;
    const double *kappa = nullptr;
    double *partial = nullptr;

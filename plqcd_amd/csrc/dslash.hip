@@ -154,7 +154,9 @@ int plqcd::dslash(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip
    if (in->nrhs != out->nrhs) return plqcd::set_error(1, "hop: RHS count mismatch");
    if (parity < PLQCD_PARITY_EVEN || parity > PLQCD_PARITY_ALL) return plqcd::set_error(1, "hop: bad parity %d", parity);
    if (mode != PLQCD_HOP_COMPAT && mode != PLQCD_HOP_WILSON) return plqcd::set_error(1, "hop: bad mode %d", mode);
-   if (op.nr < 0 || op.r0 < 0 || (op.nr > 0 && op.r0 + op.nr > in->nrhs))
+   if (op.rlist && (op.nr < 1 || op.nr > in->nrhs))
+      return plqcd::set_error(1, "hop: RHS list of %d entries for fields of %d", op.nr, in->nrhs);
+   if (!op.rlist && (op.nr < 0 || op.r0 < 0 || (op.nr > 0 && op.r0 + op.nr > in->nrhs)))
       return plqcd::set_error(1, "hop: RHS window %d + %d outside the fields' %d", op.r0, op.nr, in->nrhs);
    if (op.alpha) {
       for (const plqcd_hip_field *f : {op.fuse_p, static_cast<const plqcd_hip_field *>(op.fuse_x)})

@@ -412,6 +412,13 @@ __device__ __forceinline__ int logical_block(const HopArgs &a)
    return b;
 }
 
+// The RHS of the fields that lane-RHS r of a launch handles: r0 + r, or the
+// r-th entry of the launch's RHS list.
+__device__ __forceinline__ int rhs_of(const HopArgs &a, int r)
+{
+   return a.rlist ? a.rlist[r] : a.r0 + r;
+}
+
 // Operands of the CG-update epilogue (EPI = 2): the search direction p, the
 // iterate x and the per-RHS step alpha.
 template <class SIO>
@@ -505,11 +512,12 @@ __device__ __forceinline__ void hop_block(const SIO &in, const SO &out, const GI
    }
    const int r = threadIdx.x % a.R;
    const int js = threadIdx.x / a.R;
+   const int rr = rhs_of(a, r);
    const int j = blk * a.spb + js;
    double nrm = 0.0;
    if (js < a.spb && j < a.Vh) {
-      const SIO si = in.at_rhs(a.r0 + r);
-      const SO so = out.at_rhs(a.r0 + r);
+      const SIO si = in.at_rhs(rr);
+      const SO so = out.at_rhs(rr);
       const Nbr n = neighbours(a, p, j);
       C acc[12];
 #pragma unroll
@@ -518,10 +526,10 @@ __device__ __forceinline__ void hop_block(const SIO &in, const SO &out, const GI
       hop_dir<1, MODE, DAG>(si, g, p, j, n, acc);
       hop_dir<2, MODE, DAG>(si, g, p, j, n, acc);
       hop_dir<3, MODE, DAG>(si, g, p, j, n, acc);
-      nrm = epilogue<EPI, NORM>(si, so, a, a.r0 + r, p, j, acc, (const C *)nullptr, f);
+      nrm = epilogue<EPI, NORM>(si, so, a, rr, p, j, acc, (const C *)nullptr, f);
    }
    if (NORM) chunk_reduce<BLOCK>(nrm, js, r, a.R, p * a.nchunk + (blk * a.spb + js) / REDUCE_CHUNK,
-                                 j - (js & (REDUCE_CHUNK - 1)) < a.Vh, a.partial + (size_t)(a.r0 + r) * 2 * a.nchunk);
+                                 j - (js & (REDUCE_CHUNK - 1)) < a.Vh, a.partial + (size_t)rr * 2 * a.nchunk);
 }
 
 template <int MODE, bool DAG, int EPI, bool NORM, int BLOCK, class SIO, class GIO, class SO>
@@ -561,7 +569,7 @@ __device__ __forceinline__ void hop_block_rb(const SIO &in, const SIO &out, cons
       for (int k = 0; k < RB; k++)
 #pragma unroll
          for (int c = 0; c < 12; c++) acc[k][c] = C{0, 0};
-      const SIO si = in.at_rhs(a.r0 + gi * RB);
+      const SIO si = in.at_rhs(rhs_of(a, gi * RB));
 #define PLQCD_RB_DIR(MU)                                                                      \
    {                                                                                          \
       typename GIO::U u;                                                                      \
@@ -570,7 +578,7 @@ __device__ __forceinline__ void hop_block_rb(const SIO &in, const SIO &out, cons
       {                                                                                       \
          typename SIO::Sp f;                                                                  \
          typename SIO::Hv r;                                                                  \
-         si.at_rhs(a.r0 + gi * RB + k).load_sp(1 - p, n.up[MU], f);                           \
+         si.at_rhs(rhs_of(a, gi * RB + k)).load_sp(1 - p, n.up[MU], f);                           \
          term_u<MU, SF>(f, u, r);                                                             \
          accum<MU, SF, MODE != MODE_COMPAT>(acc[k], r);                                       \
       }                                                                                       \
@@ -579,7 +587,7 @@ __device__ __forceinline__ void hop_block_rb(const SIO &in, const SIO &out, cons
       {                                                                                       \
          typename SIO::Sp f;                                                                  \
          typename SIO::Hv r;                                                                  \
-         si.at_rhs(a.r0 + gi * RB + k).load_sp(1 - p, n.dn[MU], f);                           \
+         si.at_rhs(rhs_of(a, gi * RB + k)).load_sp(1 - p, n.dn[MU], f);                           \
          term_udag<MU, SB>(f, u, r);                                                          \
          accum<MU, SB, true>(acc[k], r);                                                      \
       }                                                                                       \
@@ -591,8 +599,8 @@ __device__ __forceinline__ void hop_block_rb(const SIO &in, const SIO &out, cons
 #undef PLQCD_RB_DIR
 #pragma unroll
       for (int k = 0; k < RB; k++)
-         nrm[k] = epilogue<EPI, NORM>(in.at_rhs(a.r0 + gi * RB + k), out.at_rhs(a.r0 + gi * RB + k), a,
-                                      a.r0 + gi * RB + k, p, j, acc[k],
+         nrm[k] = epilogue<EPI, NORM>(in.at_rhs(rhs_of(a, gi * RB + k)), out.at_rhs(rhs_of(a, gi * RB + k)), a,
+                                      rhs_of(a, gi * RB + k), p, j, acc[k],
                                       (const C *)nullptr, f);
    }
    if (NORM) {
@@ -600,7 +608,7 @@ __device__ __forceinline__ void hop_block_rb(const SIO &in, const SIO &out, cons
       for (int k = 0; k < RB; k++)
          chunk_reduce<BLOCK>(nrm[k], js, gi, G, p * a.nchunk + (blk * a.spb + js) / REDUCE_CHUNK,
                              j - (js & (REDUCE_CHUNK - 1)) < a.Vh,
-                             a.partial + (size_t)(a.r0 + gi * RB + k) * 2 * a.nchunk, k);
+                             a.partial + (size_t)(rhs_of(a, gi * RB + k)) * 2 * a.nchunk, k);
    }
 }
 
@@ -654,10 +662,11 @@ __device__ __forceinline__ void hop_tmarch_block(const SIO &in, const SO &out, c
    const int chunk = lb / nbS;
    const int r = threadIdx.x % a.R;
    const int js = threadIdx.x / a.R;
+   const int rr = rhs_of(a, r);
    const int c = sb * a.spb + js;
    const bool valid = js < a.spb && c < S;  // a batch block may have spare lanes
-   const SIO si = in.at_rhs(a.r0 + r);
-   const SO so = out.at_rhs(a.r0 + r);
+   const SIO si = in.at_rhs(rr);
+   const SO so = out.at_rhs(rr);
 
    const int t0 = chunk * a.T;
    const int cc = valid ? c : 0;
@@ -789,14 +798,14 @@ __device__ __forceinline__ void hop_tmarch_block(const SIO &in, const SO &out, c
          if (EPI >= 1) {
             C cv[12];
             SIO::to_c(cs, cv);
-            nrm = epilogue<EPI, NORM>(si, so, a, a.r0 + r, p, j, acc, cv, fuse);
+            nrm = epilogue<EPI, NORM>(si, so, a, rr, p, j, acc, cv, fuse);
          } else {
-            nrm = epilogue<EPI, NORM>(si, so, a, a.r0 + r, p, j, acc, (const C *)nullptr, fuse);
+            nrm = epilogue<EPI, NORM>(si, so, a, rr, p, j, acc, (const C *)nullptr, fuse);
          }
          put_cen(f);
       }
       if (NORM) chunk_reduce<BLOCK>(nrm, js, r, a.R, p * a.nchunk + (sb * a.spb + js + base) / REDUCE_CHUNK, valid,
-                                    a.partial + (size_t)(a.r0 + r) * 2 * a.nchunk, s);
+                                    a.partial + (size_t)rr * 2 * a.nchunk, s);
    }
 }
 
@@ -932,8 +941,9 @@ template <int MODE, bool DAG, int EPI, bool NORM, int BLOCK, class SIO, class GI
 __global__ __launch_bounds__(BLOCK) void hop_tile_kernel(SIO in, SIO out, GIO g, HopArgs a, CgFuse<SIO> fuse)
 {
    typedef typename SIO::C C;
-   in = in.at_rhs(a.r0);  // the launch's one RHS
-   out = out.at_rhs(a.r0);
+   const int r0 = rhs_of(a, 0);  // the launch's one RHS
+   in = in.at_rhs(r0);
+   out = out.at_rhs(r0);
    constexpr int SF = DAG ? 1 : -1;  // forward projector sign
    typedef typename SIO::Sp Sp;
    typedef typename SIO::Hv Hv;
@@ -1070,13 +1080,13 @@ __global__ __launch_bounds__(BLOCK) void hop_tile_kernel(SIO in, SIO out, GIO g,
 #pragma unroll
          for (int m = 0; m < Sp::N; m++) cs.e[m] = buf[m][tid];
          SIO::to_c(cs, cen);
-         nrm = epilogue<EPI, NORM>(in, out, a, a.r0, p, j, acc, cen, fuse);
+         nrm = epilogue<EPI, NORM>(in, out, a, r0, p, j, acc, cen, fuse);
       } else {
-         nrm = epilogue<EPI, NORM>(in, out, a, a.r0, p, j, acc, (const C *)nullptr, fuse);
+         nrm = epilogue<EPI, NORM>(in, out, a, r0, p, j, acc, (const C *)nullptr, fuse);
       }
       if (NORM)
          chunk_reduce<BLOCK>(nrm, lc, 0, 1, p * a.nchunk + j / REDUCE_CHUNK, true,
-                             a.partial + (size_t)a.r0 * 2 * a.nchunk, s);
+                             a.partial + (size_t)r0 * 2 * a.nchunk, s);
    }
 }
 
@@ -1144,7 +1154,8 @@ int launch_t(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip_fiel
    const bool resid = op.resid_b != nullptr;  // EPI 3: out may have another precision
    // lanes: the window of right-hand sides [op.r0, op.r0 + R) of the fields
    const int R = op.nr > 0 ? op.nr : in->nrhs;
-   a.r0 = op.nr > 0 ? op.r0 : 0;
+   a.r0 = op.nr > 0 && !op.rlist ? op.r0 : 0;
+   a.rlist = op.rlist;
    const bool solver = op.kappa != nullptr;
    HopTuning tu = plqcd::default_tuning(PREC, LINKS, R, solver);
    plqcd::lookup_tuning(plqcd::TuneKey{{lat->L[0], lat->L[1], lat->L[2], lat->L[3]}, PREC, LINKS, R,

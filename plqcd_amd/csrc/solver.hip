@@ -128,14 +128,14 @@ int CgEngine::alloc(int sloppy)
    }
    PLQCD_HIP_TRY(hipMalloc(&partial_, partial_doubles(b_) * sizeof(double)));
    PLQCD_HIP_TRY(hipMalloc(&scal_, 7 * S_ * sizeof(double)));
-   PLQCD_HIP_TRY(hipMalloc(&imask_, 2 * S_ * sizeof(int)));
+   PLQCD_HIP_TRY(hipMalloc(&imask_, 3 * S_ * sizeof(int)));
    PLQCD_HIP_TRY(hipHostMalloc(&host_, 2 * S_ * sizeof(double), hipHostMallocDefault));
-   PLQCD_HIP_TRY(hipHostMalloc(&hmask_, 2 * S_ * sizeof(int), hipHostMallocDefault));
+   PLQCD_HIP_TRY(hipHostMalloc(&hmask_, 3 * S_ * sizeof(int), hipHostMallocDefault));
    PLQCD_HIP_TRY(hipEventCreateWithFlags(&ev_, hipEventDisableTiming));
    hipStream_t st = lat_->stream;
    std::vector<double> z(7 * S_, 0.0);
    PLQCD_HIP_TRY(hipMemcpyAsync(scal_, z.data(), z.size() * sizeof(double), hipMemcpyHostToDevice, st));
-   std::vector<int> zi(2 * S_, 0);
+   std::vector<int> zi(3 * S_, 0);
    PLQCD_HIP_TRY(hipMemcpyAsync(imask_, zi.data(), zi.size() * sizeof(int), hipMemcpyHostToDevice, st));
    if (int rc = blas_zero(ps_, nullptr)) return rc;
    if (int rc = blas_zero(xs_, nullptr)) return rc;
@@ -346,24 +346,29 @@ int CgEngine::reliable_update()
    const Window w = window_of(update_);
    if (w.nr == 0) return 0;
    if (int rc = blas_accum(x_, xs_, 1, update_d(), w)) return rc;
-   for (int s = w.r0; s < w.r0 + w.nr;) {
+   // every slot being updated in one launch of M and one of the residual, over
+   // the list of those slots (begin_finish uploaded it): each link is loaded
+   // once for all of them however the slots lie in the engine
+   int nupd = 0;
+   for (int s = w.r0; s < w.r0 + w.nr; s++) nupd += update_[s] ? 1 : 0;
+   DslashOp m;
+   m.rlist = ulist_d();
+   m.nr = nupd;
+   m.kappa = kappa_d();
+   if (int rc = dslash(g_, x_, t1_, PLQCD_PARITY_ALL, PLQCD_HOP_WILSON, m)) return rc;
+   // r = b - M^+ t1 in FP64, stored straight into the iterate's residual in
+   // its own precision, with |r|^2
+   m.dag = 1;
+   m.partial = partial_;
+   m.resid_b = b_;
+   if (int rc = dslash(g_, t1_, rs_, PLQCD_PARITY_ALL, PLQCD_HOP_WILSON, m)) return rc;
+   for (int s = w.r0; s < w.r0 + w.nr;) {  // |r|^2 per run of consecutive slots
       if (!update_[s]) {
          s++;
          continue;
       }
       int e = s;
       while (e < w.r0 + w.nr && update_[e]) e++;
-      DslashOp m;
-      m.r0 = s;
-      m.nr = e - s;
-      m.kappa = kappa_d();
-      if (int rc = dslash(g_, x_, t1_, PLQCD_PARITY_ALL, PLQCD_HOP_WILSON, m)) return rc;
-      // r = b - M^+ t1 in FP64, stored straight into the iterate's residual in
-      // its own precision, with |r|^2
-      m.dag = 1;
-      m.partial = partial_;
-      m.resid_b = b_;
-      if (int rc = dslash(g_, t1_, rs_, PLQCD_PARITY_ALL, PLQCD_HOP_WILSON, m)) return rc;
       if (int rc = reduce_partials(partial_, 2 * nc, 0, 2 * nc, e - s, tmp_d(), st, s)) return rc;
       s = e;
    }
@@ -390,7 +395,9 @@ int CgEngine::begin_finish(bool &updating)
    }
    if (!nupd) return 0;
    std::copy(update_.begin(), update_.end(), hmask_ + S_);
-   PLQCD_HIP_TRY(hipMemcpyAsync(update_d(), hmask_ + S_, S_ * sizeof(int), hipMemcpyHostToDevice, st));
+   for (int s = 0, k = 0; s < S_; s++)  // the slots being updated, in order
+      if (update_[s]) hmask_[2 * S_ + k++] = s;
+   PLQCD_HIP_TRY(hipMemcpyAsync(update_d(), hmask_ + S_, 2 * S_ * sizeof(int), hipMemcpyHostToDevice, st));
    if (int rc = reliable_update()) return rc;
    PLQCD_HIP_TRY(hipEventRecord(ev_, st));
    updating = true;
