@@ -337,8 +337,8 @@ int CgEngine::launch()
 }
 
 // x += xs ; r = b - M^+M x ; rs = r ; tmp <- |r|^2, for the slots in update.
-// The FP64 operators run once per run of consecutive slots being updated
-// (their epilogue forms the residual of every RHS it covers, see EPI 3).
+// The FP64 operators run once over the list of slots being updated (the
+// residual epilogue forms the residual of every RHS it covers, see EPI 3).
 int CgEngine::reliable_update()
 {
    hipStream_t st = lat_->stream;
