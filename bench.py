@@ -358,8 +358,9 @@ def time_hop(hp, g, fi, fo, n=20, warmup=10):
 
 def multi_rhs_run(L, formats=((0, 18), (2, 12)), rhs=(1, 4, 12, 24)):
     """H on R right-hand sides sharing one gauge field: us per
-    application and per RHS, default launch shapes, 20 timed applications
-    after 10 untimed.  Returns {format: {R: {...}}}."""
+    application and per RHS, default launch shapes, 100 timed applications
+    after 10 untimed (20 read about 3 % high while the clocks settle).
+    Returns {format: {R: {...}}}."""
     from plqcd_amd import hopping as hp
     V = int(np.prod(L))
     u = hp.random_gauge_su3(V, 1)
@@ -372,7 +373,7 @@ def multi_rhs_run(L, formats=((0, 18), (2, 12)), rhs=(1, 4, 12, 24)):
             fi, fo = hp.Field(lat, prec, R), hp.Field(lat, prec, R)
             for r in range(R):
                 fi.upload(hp.random_spinor(V, 10 + r), r)
-            ms = time_hop(hp, g, fi, fo)
+            ms = time_hop(hp, g, fi, fo, n=100)
             rec[f"R{R}"] = {"us_per_application": round(ms * 1e3, 1), "us_per_rhs": round(ms * 1e3 / R, 1),
                             "gflops": round(FLOP_PER_SITE * V * R / ms / 1e6, 1)}
             fi.close(); fo.close()
