@@ -78,9 +78,11 @@ extern "C" int plqcd_hip_autotune(const plqcd_hip_gauge *g, const plqcd_hip_fiel
    const bool had = plqcd::lookup_tuning(key, &prev);
    double best_t = 1e300;
    HopTuning best_c = plqcd::default_tuning(g->prec, g->links, in->nrhs);
-   hipEvent_t e0, e1;
-   PLQCD_HIP_TRY(hipEventCreate(&e0));
-   PLQCD_HIP_TRY(hipEventCreate(&e1));
+   hipEvent_t e0 = nullptr, e1 = nullptr;
+   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+      if (e0) (void)hipEventDestroy(e0);
+      return plqcd::set_error(10, "autotune: hipEventCreate");
+   }
    int rc = 0;
    for (const HopTuning &c : cands) {
       plqcd::store_tuning(key, c);
@@ -205,9 +207,10 @@ extern "C" int plqcd_hip_autotune_solver(const plqcd_hip_gauge *g, const plqcd_h
       }
       if (!rc && best_t == 1e300) rc = plqcd::set_error(2, "autotune_solver: no launch shape applies");
    }
-   if (!rc) {
+   {
+      // also after a failed launch: launches already queued still use the scratch freed below
       const hipError_t e = hipStreamSynchronize(lat->stream);
-      if (e != hipSuccess) rc = plqcd::set_error(10, "autotune_solver: %s", hipGetErrorString(e));
+      if (!rc && e != hipSuccess) rc = plqcd::set_error(10, "autotune_solver: %s", hipGetErrorString(e));
    }
    if (e0) (void)hipEventDestroy(e0);
    if (e1) (void)hipEventDestroy(e1);
