@@ -358,7 +358,7 @@ int plqcd_hip_hop_get_tuning(int prec, int out[7]);
 int plqcd_hip_autotune(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip_field *out, int reps,
                        int best[7], double *best_ms);
 /* The same for the solvers' launches over all RHS of `in` (M = 1 - kappa H
- * with |Mp|^2, then M^+ fused with the CG update and |r|^2, and the residual
+ * with |Mp|^2, then M^+ fused with the CG residual update and |r|^2, and the residual
  * launch of reliable updates): times the pair of launches of one CG iteration
  * with `reps` timed pairs per shape, on scratch fields it allocates (`in`,
  * whose storage precision must be the gauge field's, is only read), and

@@ -16,8 +16,13 @@ struct Window {
 size_t partial_doubles(const plqcd_hip_field *f);
 int blas_norm_finish(const plqcd_hip_field *f, double *partial, double *out);  // overwrites partial
 
-// p = r + beta p
-int blas_xpay(const plqcd_hip_field *r, const double *beta, plqcd_hip_field *p, const Window &w = Window{});
+// x += alpha p in the working precision of the storage (float for F32/F16,
+// double for F64), then p = r + beta p in double, from one read of p
+int blas_cg_step(const plqcd_hip_field *r, const double *beta, plqcd_hip_field *p, const double *alpha,
+                 plqcd_hip_field *x, const Window &w = Window{});
+// x += alpha p as in blas_cg_step, for the RHS with mask[r] != 0 only (mask required)
+int blas_x_step(const double *alpha, const plqcd_hip_field *p, plqcd_hip_field *x, const int *mask,
+                const Window &w = Window{});
 // The masked kernels below touch only the RHS r with mask[r] != 0 (all when
 // mask is null); the chunk sums of untouched RHS are written as 0.  With a
 // window, RHS outside it are not touched at all (their chunk sums neither).

@@ -1,5 +1,6 @@
 // Site-wise vector kernels for the solvers: every operand may be stored in
-// any precision; arithmetic is done in double.  Scalars are per right-hand
+// any precision; arithmetic is done in double, except the CG iterate update
+// x += alpha p, which runs in the storage's working precision.  Scalars are per right-hand
 // side and live in device memory, so a solver iteration needs no host round
 // trip.  Sums use the deterministic chunk reduction of reduce.h.
 #include "blas.h"
@@ -35,20 +36,54 @@ __device__ __forceinline__ void blas_reduce(const BlasArgs &a, double v, int p, 
                         a.partial + (size_t)r * 2 * a.nchunk);
 }
 
-// p = r + beta p
+// x += alpha p at one site, in the storage's working precision (float for
+// F32/F16, double for F64): the CG iterate update
 template <int P>
-__global__ __launch_bounds__(kBlock) void k_xpay(SpinorIO<P> rv, const double *beta, SpinorIO<P> pv, BlasArgs a)
+__device__ __forceinline__ void x_step(const SpinorIO<P> &X, const typename SpinorIO<P>::C ps[12], double alpha, int p,
+                                       int j)
+{
+   typedef typename SpinorIO<P>::C C;
+   typedef typename SpinorIO<P>::real real;
+   const real al = (real)alpha;
+   C xs[12];
+   X.load(p, j, xs);
+#pragma unroll
+   for (int m = 0; m < 12; m++) xs[m] = C{fma(al, ps[m].x, xs[m].x), fma(al, ps[m].y, xs[m].y)};
+   X.store(p, j, xs);
+}
+
+// x += alpha p (working precision), then p = r + beta p (double), from one
+// read of p: the iterate update of the last CG iteration, deferred to this
+// pass, and the new search direction
+template <int P>
+__global__ __launch_bounds__(kBlock) void k_cg_step(SpinorIO<P> rv, const double *beta, SpinorIO<P> pv,
+                                                    const double *alpha, SpinorIO<P> xv, BlasArgs a)
 {
    int p, j, js, r;
    if (!blas_site(a, p, j, js, r)) return;
-   const double be = beta[r];
    const SpinorIO<P> Rr = rv.at_rhs(r), Pp = pv.at_rhs(r);
-   d2 rs[12], ps[12];
+   typename SpinorIO<P>::C ps[12];
+   Pp.load(p, j, ps);
+   x_step(xv.at_rhs(r), ps, alpha[r], p, j);
+   const double be = beta[r];
+   d2 rs[12], pn[12];
    load_d(Rr, p, j, rs);
-   load_d(Pp, p, j, ps);
 #pragma unroll
-   for (int c = 0; c < 12; c++) ps[c] = d2{fma(be, ps[c].x, rs[c].x), fma(be, ps[c].y, rs[c].y)};
-   store_d(Pp, p, j, ps);
+   for (int c = 0; c < 12; c++)
+      pn[c] = d2{fma(be, (double)ps[c].x, rs[c].x), fma(be, (double)ps[c].y, rs[c].y)};
+   store_d(Pp, p, j, pn);
+}
+
+// x += alpha p (working precision) for the right-hand sides with mask[r] != 0
+template <int P>
+__global__ __launch_bounds__(kBlock) void k_x_step(const double *alpha, SpinorIO<P> pv, SpinorIO<P> xv, const int *mask,
+                                                   BlasArgs a)
+{
+   int p, j, js, r;
+   if (!blas_site(a, p, j, js, r) || !mask[r]) return;
+   typename SpinorIO<P>::C ps[12];
+   pv.at_rhs(r).load(p, j, ps);
+   x_step(xv.at_rhs(r), ps, alpha[r], p, j);
 }
 
 // r = b - t ; |r|^2
@@ -249,15 +284,32 @@ int blas_norm_finish(const plqcd_hip_field *f, double *partial, double *out)
    return reduce_partials(partial, (size_t)2 * nc, 0, 2 * nc, f->nrhs, out, f->lat->stream);
 }
 
-int blas_xpay(const plqcd_hip_field *r, const double *beta, plqcd_hip_field *p, const Window &w)
+int blas_cg_step(const plqcd_hip_field *r, const double *beta, plqcd_hip_field *p, const double *alpha,
+                 plqcd_hip_field *x, const Window &w)
 {
-   if (int rc = check_pair(r, p, w)) return rc;
-   if (r->prec != p->prec) return set_error(1, "blas_xpay: precision mismatch");
+   if (int rc = check_pair(r, p, w) | check_pair(r, x, w)) return rc;
+   if (r->prec != p->prec || r->prec != x->prec) return set_error(1, "blas_cg_step: precision mismatch");
+   if (p == x || p == r) return set_error(1, "blas_cg_step: p must be distinct from r and x");
    const BlasArgs a = make_args(r, nullptr, w);
    hipStream_t st = r->lat->stream;
    return dispatch1(r->prec, [&](auto P) {
       constexpr int PP = decltype(P)::value;
-      hipLaunchKernelGGL(k_xpay<PP>, grid_of(a), block_of(a), 0, st, spinor_io<PP>(r), beta, spinor_io<PP>(p), a);
+      hipLaunchKernelGGL(k_cg_step<PP>, grid_of(a), block_of(a), 0, st, spinor_io<PP>(r), beta, spinor_io<PP>(p),
+                         alpha, spinor_io<PP>(x), a);
+   });
+}
+
+int blas_x_step(const double *alpha, const plqcd_hip_field *p, plqcd_hip_field *x, const int *mask, const Window &w)
+{
+   if (int rc = check_pair(p, x, w)) return rc;
+   if (p->prec != x->prec) return set_error(1, "blas_x_step: precision mismatch");
+   if (p == x || !mask) return set_error(1, "blas_x_step: distinct p and x and a mask expected");
+   const BlasArgs a = make_args(p, nullptr, w);
+   hipStream_t st = p->lat->stream;
+   return dispatch1(p->prec, [&](auto P) {
+      constexpr int PP = decltype(P)::value;
+      hipLaunchKernelGGL(k_x_step<PP>, grid_of(a), block_of(a), 0, st, alpha, spinor_io<PP>(p), spinor_io<PP>(x),
+                         mask, a);
    });
 }
 

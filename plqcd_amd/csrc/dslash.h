@@ -50,9 +50,9 @@ namespace plqcd {
 //   kappa   != null: out = in - kappa[r] * (H in)  (device array, one per RHS)
 //   dag     != 0:    H is replaced by H^+ = gamma5 H gamma5
 //   partial != null: chunk sums of |out|^2 into partial[r][2*nchunk]
-//   alpha   != null: CG update (with dag, kappa and partial): Ap = in - kappa H^+ in
-//                    is not stored; fuse_x += alpha[r] fuse_p and out -= alpha[r] Ap
-//                    (out holds the residual), whose |out|^2 goes to partial
+//   alpha   != null: CG residual update (with dag, kappa and partial): Ap = in -
+//                    kappa H^+ in is not stored; out -= alpha[r] Ap (out holds
+//                    the residual), whose |out|^2 goes to partial
 //   resid_b != null: residual (with dag, kappa and partial; F64 storage):
 //                    out = resid_b - (in - kappa H^+ in) for every RHS of the
 //                    launch, stored in out's own precision (any), and its |out|^2
@@ -68,8 +68,6 @@ struct DslashOp {
    const double *kappa = nullptr;
    double *partial = nullptr;
    const double *alpha = nullptr;
-   const plqcd_hip_field *fuse_p = nullptr;
-   plqcd_hip_field *fuse_x = nullptr;
    const plqcd_hip_field *resid_b = nullptr;
 };
 

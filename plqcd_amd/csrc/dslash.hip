@@ -158,12 +158,6 @@ int plqcd::dslash(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip
       return plqcd::set_error(1, "hop: RHS list of %d entries for fields of %d", op.nr, in->nrhs);
    if (!op.rlist && (op.nr < 0 || op.r0 < 0 || (op.nr > 0 && op.r0 + op.nr > in->nrhs)))
       return plqcd::set_error(1, "hop: RHS window %d + %d outside the fields' %d", op.r0, op.nr, in->nrhs);
-   if (op.alpha) {
-      for (const plqcd_hip_field *f : {op.fuse_p, static_cast<const plqcd_hip_field *>(op.fuse_x)})
-         if (!f || f->lat != g->lat || f->prec != g->prec || f->nrhs != in->nrhs || f == in || f == out)
-            return plqcd::set_error(1, "hop: CG-update operands must be distinct fields like in");
-      if (op.fuse_p == op.fuse_x) return plqcd::set_error(1, "hop: CG-update p and x must be distinct");
-   }
    if (op.resid_b) {
       const plqcd_hip_field *b = op.resid_b;
       if (g->prec != PLQCD_PREC_F64 || b->lat != g->lat || b->prec != PLQCD_PREC_F64 || b->nrhs != in->nrhs ||

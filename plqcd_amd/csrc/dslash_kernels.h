@@ -419,19 +419,20 @@ __device__ __forceinline__ int rhs_of(const HopArgs &a, int r)
    return a.rlist ? a.rlist[r] : a.r0 + r;
 }
 
-// Operands of the CG-update epilogue (EPI = 2): the search direction p, the
-// iterate x and the per-RHS step alpha.
+// Operands of the epilogues: the per-RHS step alpha (EPI = 2) and the
+// right-hand side b (EPI = 3).
 template <class SIO>
 struct CgFuse {
-   SIO p, x;  // EPI 3: p holds b
+   SIO p;  // EPI 3: b
    const double *alpha;
 };
 
 // Epilogue: EPI = 0 stores H in; EPI = 1 stores in(x) - kappa_r * (H in)(x).
 // EPI = 2 forms Ap = in(x) - kappa_r * (H in)(x) in registers and applies the
-// CG update at the site instead of storing it: x += alpha_r p, out -= alpha_r
-// Ap (out holds the residual), in the working precision of the kernel (float
-// for F32/F16 storage, double for F64).
+// CG residual update at the site instead of storing it: out -= alpha_r Ap (out
+// holds the residual), in the working precision of the kernel (float for
+// F32/F16 storage, double for F64).  The iterate's update x += alpha_r p is
+// left to the solver's next pass over p.
 // EPI = 3 (F64 storage) stores the residual b - Ap, with b in f.p, in the
 // precision of `out`, for every RHS of the launch (the reliable update of the
 // mixed-precision CG: the true residual goes straight into the low-precision
@@ -467,14 +468,6 @@ __device__ __forceinline__ double epilogue(const SIO &in, const SO &out, const H
       return NORM ? norm2(rs) : 0.0;
    } else if constexpr (EPI == 2) {
       const real al = (real)f.alpha[r];
-      {
-         C ps[12], xs[12];
-         f.p.at_rhs(r).load(p, j, ps);
-         f.x.at_rhs(r).load(p, j, xs);
-#pragma unroll
-         for (int m = 0; m < 12; m++) xs[m] = C{fma(al, ps[m].x, xs[m].x), fma(al, ps[m].y, xs[m].y)};
-         f.x.at_rhs(r).store(p, j, xs);
-      }
       C rs[12];
       out.load(p, j, rs);
       double nrm = 0.0;
@@ -1185,14 +1178,12 @@ int launch_t(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip_fiel
    const bool norm = op.partial != nullptr;
    if ((dag || epi || norm) && mode != PLQCD_HOP_WILSON)
       return plqcd::set_error(2, "hop: M, M^+ and norms need the wilson mode");
-   if (epi == 2 && (!dag || !norm || !op.kappa || !op.fuse_p || !op.fuse_x))
-      return plqcd::set_error(2, "hop: the CG-update epilogue needs M^+, a norm, kappa, p and x");
+   if (epi == 2 && (!dag || !norm || !op.kappa))
+      return plqcd::set_error(2, "hop: the CG-update epilogue needs M^+, a norm and kappa");
    if (epi == 3 && (PREC != PLQCD_PREC_F64 || !dag || !norm || !op.kappa))
       return plqcd::set_error(2, "hop: the residual epilogue needs F64 storage, M^+, a norm and kappa");
    CgFuse<S> fu{};
    if (epi == 2) {
-      fu.p = spinor_io<PREC>(op.fuse_p);
-      fu.x = spinor_io<PREC>(op.fuse_x);
       fu.alpha = op.alpha;
    } else if (epi == 3) {
       fu.p = spinor_io<PREC>(op.resid_b);

@@ -11,11 +11,16 @@
 // replaces the iterated one.  A slot stops when its FP64 true residual meets
 // its tolerance.
 //
-// One iteration is a fixed sequence of kernels (beta, p update, M p with
-// |Mp|^2, alpha, then M^+ (Mp) with the x/r update and |r|^2 fused into its
-// epilogue, so Ap is never stored) whose scalars live on the
-// device; it is captured once into a hipGraph and replayed.  Only |r|^2 comes
-// back to the host each iteration, to decide reliable updates.
+// One iteration is a fixed sequence of kernels whose scalars live on the
+// device: beta; one pass that applies the previous iteration's x += alpha p
+// and forms p = r + beta p; M p with |Mp|^2; alpha; then M^+ (Mp) with the
+// residual update r -= alpha Ap and |r|^2 fused into its epilogue, so Ap is
+// never stored.  The iterate therefore trails by one step: a reliable update
+// first applies the pending x += alpha p of the slots it updates and zeroes
+// their alpha, so the next pass adds nothing twice.  The arithmetic of each
+// update is unchanged, and so are the iterates.  The sequence is captured
+// once into a hipGraph and replayed; only |r|^2 comes back to the host each
+// iteration, to decide reliable updates.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -46,11 +51,22 @@ __global__ void k_beta(double *rr, const double *rr_new, const int *active, doub
    }
 }
 
-// rr[s] = rr_new[s] = src[s] for the slots in mask
-__global__ void k_admit(const int *mask, const double *src, double *rr, double *rr_new, int S)
+// rr[s] = rr_new[s] = src[s] and alpha[s] = 0 (no pending iterate update)
+// for the slots in mask
+__global__ void k_admit(const int *mask, const double *src, double *rr, double *rr_new, double *alpha, int S)
 {
    const int r = threadIdx.x;
-   if (r < S && mask[r]) rr[r] = rr_new[r] = src[r];
+   if (r < S && mask[r]) {
+      rr[r] = rr_new[r] = src[r];
+      alpha[r] = 0.0;
+   }
+}
+
+// v[s] = 0 for the slots in mask
+__global__ void k_clear(const int *mask, double *v, int S)
+{
+   const int r = threadIdx.x;
+   if (r < S && mask[r]) v[r] = 0.0;
 }
 
 }  // namespace
@@ -204,7 +220,8 @@ int CgEngine::admit(const std::vector<int> &which, const double *kappa, const do
    if (int rc = blas_zero(ps_, update_d(), w)) return rc;
    if (int rc = blas_copy(rs_, b_, partial_, update_d(), w)) return rc;
    if (int rc = reduce_partials(partial_, 2 * nc, 0, 2 * nc, w.nr, tmp_d(), st, w.r0)) return rc;
-   hipLaunchKernelGGL(k_admit, dim3(1), dim3(kMaxSlots), 0, st, update_d(), tmp_d(), rr_d(), rr_new_d(), S_);
+   hipLaunchKernelGGL(k_admit, dim3(1), dim3(kMaxSlots), 0, st, update_d(), tmp_d(), rr_d(), rr_new_d(), alpha_d(),
+                      S_);
    PLQCD_HIP_TRY(hipGetLastError());
    PLQCD_HIP_TRY(hipMemcpyAsync(host_ + S_, tmp_d(), S_ * sizeof(double), hipMemcpyDeviceToHost, st));
    PLQCD_HIP_TRY(hipStreamSynchronize(st));
@@ -251,7 +268,8 @@ int CgEngine::iteration(int nr)
    const int nc = reduce_chunks(lat_->Vh);
    const Window w{0, nr};
    hipLaunchKernelGGL(k_beta, dim3(1), dim3(kMaxSlots), 0, st, rr_d(), rr_new_d(), active_d(), beta_d(), S_);
-   if (int rc = blas_xpay(rs_, beta_d(), ps_, w)) return rc;
+   // xs += alpha ps (alpha of the last iteration), then ps = rs + beta ps
+   if (int rc = blas_cg_step(rs_, beta_d(), ps_, alpha_d(), xs_, w)) return rc;
    DslashOp m;
    m.nr = nr;
    m.kappa = kappa_d();
@@ -259,15 +277,13 @@ int CgEngine::iteration(int nr)
    if (int rc = dslash(gs_, ps_, qs_, PLQCD_PARITY_ALL, PLQCD_HOP_WILSON, m)) return rc;
    if (int rc = reduce_partials(partial_, 2 * nc, 0, 2 * nc, nr, pap_d(), st)) return rc;
    hipLaunchKernelGGL(k_alpha, dim3(1), dim3(kMaxSlots), 0, st, rr_d(), pap_d(), active_d(), alpha_d(), S_);
-   // Ap = M^+ q, fused with xs += alpha p, rs -= alpha Ap and |rs|^2
+   // Ap = M^+ q, fused with rs -= alpha Ap and |rs|^2
    DslashOp md;
    md.nr = nr;
    md.kappa = kappa_d();
    md.dag = 1;
    md.partial = partial_;
    md.alpha = alpha_d();
-   md.fuse_p = ps_;
-   md.fuse_x = xs_;
    if (int rc = dslash(gs_, qs_, rs_, PLQCD_PARITY_ALL, PLQCD_HOP_WILSON, md)) return rc;
    if (int rc = reduce_partials(partial_, 2 * nc, 0, 2 * nc, nr, rr_new_d(), st)) return rc;
    PLQCD_HIP_TRY(hipGetLastError());
@@ -336,7 +352,8 @@ int CgEngine::launch()
    return 0;
 }
 
-// x += xs ; r = b - M^+M x ; rs = r ; tmp <- |r|^2, for the slots in update.
+// xs += alpha ps (pending) ; x += xs ; r = b - M^+M x ; rs = r ; tmp <- |r|^2,
+// for the slots in update.
 // The FP64 operators run once over the list of slots being updated (the
 // residual epilogue forms the residual of every RHS it covers, see EPI 3).
 int CgEngine::reliable_update()
@@ -345,6 +362,11 @@ int CgEngine::reliable_update()
    const int nc = reduce_chunks(lat_->Vh);
    const Window w = window_of(update_);
    if (w.nr == 0) return 0;
+   // the pending xs += alpha ps of the last iteration, then alpha = 0 so
+   // that the next iteration does not apply it again
+   if (int rc = blas_x_step(alpha_d(), ps_, xs_, update_d(), w)) return rc;
+   hipLaunchKernelGGL(k_clear, dim3(1), dim3(kMaxSlots), 0, st, update_d(), alpha_d(), S_);
+   PLQCD_HIP_TRY(hipGetLastError());
    if (int rc = blas_accum(x_, xs_, 1, update_d(), w)) return rc;
    // every slot being updated in one launch of M and one of the residual, over
    // the list of those slots (begin_finish uploaded it): each link is loaded

@@ -10,7 +10,7 @@
 // cache that every later launch of that shape consults, and return it.
 // plqcd_hip_autotune times plain applications of H; plqcd_hip_autotune_solver
 // times the pair of launches of one CG iteration (M p with |q|^2, then M^+ q
-// fused with the x and r updates and |r|^2), whose shapes are cached apart.
+// fused with the residual update and |r|^2), whose shapes are cached apart.
 #include <vector>
 
 #include "blas.h"
@@ -145,13 +145,12 @@ extern "C" int plqcd_hip_autotune_solver(const plqcd_hip_gauge *g, const plqcd_h
    const int R = in->nrhs;
    const plqcd::TuneKey key{{lat->L[0], lat->L[1], lat->L[2], lat->L[3]}, g->prec, g->links, R,
                             plqcd::kTuneOpSolver};
-   // scratch: q = M p, the residual r and iterate x of the fused update, kappa
-   // and alpha per RHS, and the chunk sums; `in` plays p and is only read
-   plqcd_hip_field *q = nullptr, *r = nullptr, *x = nullptr;
+   // scratch: q = M p, the residual r of the fused update, kappa and alpha
+   // per RHS, and the chunk sums; `in` plays p and is only read
+   plqcd_hip_field *q = nullptr, *r = nullptr;
    double *dk = nullptr, *partial = nullptr;
    int rc = plqcd_hip_field_create(lat, in->prec, R, &q);
    if (!rc) rc = plqcd_hip_field_create(lat, in->prec, R, &r);
-   if (!rc) rc = plqcd_hip_field_create(lat, in->prec, R, &x);
    if (!rc) {
       std::vector<double> ka(2 * size_t(R), 0.0);  // kappa 0.124 (a typical Wilson value), alpha 0
       for (int k = 0; k < R; k++) ka[k] = 0.124;
@@ -159,19 +158,16 @@ extern "C" int plqcd_hip_autotune_solver(const plqcd_hip_gauge *g, const plqcd_h
       if (e == hipSuccess) e = hipMalloc(&partial, plqcd::partial_doubles(in) * sizeof(double));
       if (e == hipSuccess) e = hipMemcpy(dk, ka.data(), ka.size() * sizeof(double), hipMemcpyHostToDevice);
       if (e == hipSuccess) e = hipMemset(r->data, 0, r->bytes);
-      if (e == hipSuccess) e = hipMemset(x->data, 0, x->bytes);
       if (e != hipSuccess) rc = plqcd::set_error(10, "autotune_solver: %s", hipGetErrorString(e));
    }
    plqcd::DslashOp m;  // q = M p and |q|^2
    m.kappa = dk;
    m.partial = partial;
-   plqcd::DslashOp md;  // M^+ q fused with x += alpha p, r -= alpha M^+ q and |r|^2
+   plqcd::DslashOp md;  // M^+ q fused with r -= alpha M^+ q and |r|^2
    md.kappa = dk;
    md.dag = 1;
    md.partial = partial;
    md.alpha = dk + R;
-   md.fuse_p = in;
-   md.fuse_x = x;
    auto pair = [&]() {
       int c = plqcd::dslash(g, in, q, PLQCD_PARITY_ALL, PLQCD_HOP_WILSON, m);
       if (!c) c = plqcd::dslash(g, q, r, PLQCD_PARITY_ALL, PLQCD_HOP_WILSON, md);
@@ -218,7 +214,6 @@ extern "C" int plqcd_hip_autotune_solver(const plqcd_hip_gauge *g, const plqcd_h
    if (partial) (void)hipFree(partial);
    plqcd_hip_field_destroy(q);
    plqcd_hip_field_destroy(r);
-   plqcd_hip_field_destroy(x);
    if (rc) {
       if (had) plqcd::store_tuning(key, prev);
       else plqcd::clear_tuning(&key);
