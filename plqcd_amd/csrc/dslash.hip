@@ -41,11 +41,11 @@ const FormatShape kOneRhs[6] = {{PLQCD_PREC_F64, 18, {1, 1, 256, 32, 1, 1, 4}},
 // FP64 one-RHS solve took 0.0806 against 0.0769 s).
 const FormatShape kOneRhsSolver[1] = {{PLQCD_PREC_F64, 18, {1, 1, 256, 32, 1, 4, 1}}};
 const HopTuning kMultiRhs[3] = {{2, 8, 256, 16, 1, 0, 0}, {2, 8, 256, 8, 1, 0, 0}, {2, 8, 256, 8, 1, 0, 0}};
-// Several RHS with the solver epilogues (M, and M^+ with the CG update): the
-// F16 gather kernel, whose 64-72 VGPRs leave room for more waves to overlap
-// the epilogue's streams (12-RHS FP16 CG at 32^3 x 64: 0.264 s, with the
-// t-march 0.306 s).
-const HopTuning kMultiRhsSolver[3] = {{2, 8, 256, 16, 1, 0, 0}, {2, 8, 256, 4, 1, 0, 0}, {2, 8, 256, 0, 1, 0, 0}};
+// Several RHS with the solver epilogues (M, and M^+ with the residual
+// update): short t-marches.  F16 with 4-slice marches: the 12-RHS mixed CG at
+// 32^3 x 64 takes 0.2515 s against 0.256 s with the gather kernel, 1.24
+// against 1.36-1.40 s at 48^3 x 96.
+const HopTuning kMultiRhsSolver[3] = {{2, 8, 256, 16, 1, 0, 0}, {2, 8, 256, 4, 1, 0, 0}, {2, 8, 256, 4, 1, 0, 0}};
 bool g_tuning_set[3] = {false, false, false};  // set by plqcd_hip_hop_tuning
 
 HopTuning one_rhs_shape(int prec, int links)
