@@ -59,7 +59,10 @@ def test_cg_true_residual(hp, sloppy):
 
 
 def test_cg_batch_equals_alone_bitwise(hp):
-    """Each RHS follows the same iterates alone and in a batch."""
+    """Each RHS follows the same iterates alone and in a batch, and each meets
+    its own tolerance by the oracle's true residual (the slots take reliable
+    updates and stop at different iterations)."""
+    from oracle import solver as osv
     from plqcd_amd import solver as sv
     L = (8, 4, 4, 8)
     kaps, tols = [0.12, 0.124, 0.1245], [1e-8, 1e-10, 1e-12]
@@ -68,6 +71,10 @@ def test_cg_batch_equals_alone_bitwise(hp):
     for sloppy in (None, gs):
         x = hp.Field(lat, nrhs=3)
         st = sv.cg_solve(g, b, x, kaps, tols, sloppy=sloppy)
+        for r in range(3):
+            assert st[r]["converged"]
+            res = osv.true_residual(bs[r], x.download(r), u, L, kaps[r])
+            assert res <= 1.05 * tols[r], (r, res)
         for r in range(3):
             b1 = hp.Field(lat).upload(bs[r])
             x1 = hp.Field(lat)
