@@ -16,11 +16,15 @@
 // structure-of-arrays.  An accessor is bound to one RHS by at_rhs(r).
 
 // Addressing: the wave-uniform part of an address (field base, parity,
-// component, direction) is formed in scalar registers; each lane adds one
-// 32-bit byte offset per site, (j*R + r)*16.  The backend then issues
-// global loads in the saddr + 32-bit voffset form with no per-component
-// vector arithmetic.  (A parity block of one component of a field must stay
-// below 4 GiB: Vh*R*16 bytes.)
+// component, direction) plus one 32-bit byte offset per lane and site,
+// (j*R + r)*16.  The compiler often folds the lane offset into the
+// per-component bases and issues 64-bit per-lane addresses (one v_mad_u64_u32
+// per load).  Forcing every base into scalar registers (readfirstlane),
+// which gives the saddr + voffset form, cut the FP64 tile kernel's vector
+// instructions by a third but made it slower: 422 against 404 us at
+// 32^3 x 64 (FP16 18-real tiles 112 against 119 us, the rest within 3 %).
+// So the loads stay as written.  (A parity block of one component of a field
+// must stay below 4 GiB: Vh*R*16 bytes.)
 
 // NT: a non-temporal (streaming) load, for data that no other lane or block
 // reads again soon; it leaves more of the L2 to data that is re-read.
