@@ -1,7 +1,7 @@
 // Vector kernels of the solvers (device fields, per-RHS device scalars).
 // All are asynchronous on the field's lattice stream.  `partial`, when not
-// null, receives the chunk sums of the named norm (partial_doubles() doubles);
-// blas_norm_finish() turns them into one double per RHS.
+// null, receives the chunk sums of the named norm (partial_doubles() doubles),
+// which reduce_partials() (reduce.h) turns into one double per RHS.
 #pragma once
 
 #include "device_common.h"
@@ -14,7 +14,6 @@ struct Window {
 };
 
 size_t partial_doubles(const plqcd_hip_field *f);
-int blas_norm_finish(const plqcd_hip_field *f, double *partial, double *out);  // overwrites partial
 
 // x += alpha p in the working precision of the storage (float for F32/F16,
 // double for F64), then p = r + beta p in double, from one read of p
@@ -26,9 +25,6 @@ int blas_x_step(const double *alpha, const plqcd_hip_field *p, plqcd_hip_field *
 // The masked kernels below touch only the RHS r with mask[r] != 0 (all when
 // mask is null); the chunk sums of untouched RHS are written as 0.  With a
 // window, RHS outside it are not touched at all (their chunk sums neither).
-// r = b - t ; partial <- |r|^2   (F64)
-int blas_residual(const plqcd_hip_field *b, const plqcd_hip_field *t, plqcd_hip_field *r, double *partial,
-                  const int *mask, const Window &w = Window{});
 // y += x ; x = 0 if zero_x   (any precisions)
 int blas_accum(plqcd_hip_field *y, plqcd_hip_field *x, int zero_x, const int *mask, const Window &w = Window{});
 // y = x ; partial <- |y|^2   (any precisions)

@@ -86,25 +86,6 @@ __global__ __launch_bounds__(kBlock) void k_x_step(const double *alpha, SpinorIO
    x_step(xv.at_rhs(r), ps, alpha[r], p, j);
 }
 
-// r = b - t ; |r|^2
-template <int PB, int PT, int PR>
-__global__ __launch_bounds__(kBlock) void k_residual(SpinorIO<PB> bv, SpinorIO<PT> tv, SpinorIO<PR> rv, const int *mask,
-                                                     BlasArgs a)
-{
-   int p, j, js, r;
-   double n = 0.0;
-   if (blas_site(a, p, j, js, r) && (!mask || mask[r])) {
-      d2 bs[12], ts[12];
-      load_d(bv.at_rhs(r), p, j, bs);
-      load_d(tv.at_rhs(r), p, j, ts);
-#pragma unroll
-      for (int c = 0; c < 12; c++) bs[c] -= ts[c];
-      store_d(rv.at_rhs(r), p, j, bs);
-      n = norm2(bs);
-   }
-   if (a.partial) blas_reduce(a, n, p, j, js, r);
-}
-
 // y += x ; x = 0 (when zero_x) -- folds a low-precision partial solution into
 // the high-precision one
 template <int PY, int PX>
@@ -278,12 +259,6 @@ int reduce_partials(double *partial, size_t stride, int first, int n, int R, dou
 
 size_t partial_doubles(const plqcd_hip_field *f) { return (size_t)f->nrhs * 2 * reduce_chunks(f->lat->Vh); }
 
-int blas_norm_finish(const plqcd_hip_field *f, double *partial, double *out)
-{
-   const int nc = reduce_chunks(f->lat->Vh);
-   return reduce_partials(partial, (size_t)2 * nc, 0, 2 * nc, f->nrhs, out, f->lat->stream);
-}
-
 int blas_cg_step(const plqcd_hip_field *r, const double *beta, plqcd_hip_field *p, const double *alpha,
                  plqcd_hip_field *x, const Window &w)
 {
@@ -311,20 +286,6 @@ int blas_x_step(const double *alpha, const plqcd_hip_field *p, plqcd_hip_field *
       hipLaunchKernelGGL(k_x_step<PP>, grid_of(a), block_of(a), 0, st, alpha, spinor_io<PP>(p), spinor_io<PP>(x),
                          mask, a);
    });
-}
-
-int blas_residual(const plqcd_hip_field *b, const plqcd_hip_field *t, plqcd_hip_field *r, double *partial,
-                  const int *mask, const Window &w)
-{
-   if (int rc = check_pair(b, t, w) | check_pair(b, r, w)) return rc;
-   if (b->prec != PLQCD_PREC_F64 || t->prec != PLQCD_PREC_F64 || r->prec != PLQCD_PREC_F64)
-      return set_error(1, "blas_residual: F64 fields expected");
-   const BlasArgs a = make_args(b, partial, w);
-   hipLaunchKernelGGL((k_residual<PLQCD_PREC_F64, PLQCD_PREC_F64, PLQCD_PREC_F64>), grid_of(a), block_of(a), 0,
-                      b->lat->stream, spinor_io<PLQCD_PREC_F64>(b), spinor_io<PLQCD_PREC_F64>(t),
-                      spinor_io<PLQCD_PREC_F64>(r), mask, a);
-   PLQCD_HIP_TRY(hipGetLastError());
-   return 0;
 }
 
 int blas_accum(plqcd_hip_field *y, plqcd_hip_field *x, int zero_x, const int *mask, const Window &w)
