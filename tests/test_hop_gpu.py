@@ -151,7 +151,9 @@ def test_device_parities(hp, orc, mode):
         assert np.array_equal(out[other], sentinel[other])
 
 
-def test_multi_rhs_field_equals_single(hp):
+def test_multi_rhs_field_equals_single(hp, orc):
+    """Each RHS of a 3-RHS field: bitwise its one-RHS run, and the oracle's
+    wilson H to 1e-13 per site."""
     L = (4, 4, 4, 8)
     V = int(np.prod(L))
     _, u = inputs(hp, L)
@@ -164,7 +166,9 @@ def test_multi_rhs_field_equals_single(hp):
         fi.upload(q, r)
     hp.hop(g, fi, fo, hp.PARITY_ALL, hp.HOP_WILSON)
     for r, q in enumerate(qs):
-        assert np.array_equal(fo.download(r), device_hop(hp, L, q, u, hp.HOP_WILSON))
+        out = fo.download(r)
+        assert np.array_equal(out, device_hop(hp, L, q, u, hp.HOP_WILSON))
+        assert site_rel_err(out, orc.hopping_full(q, u, L, orc.MODE_WILSON)) < TOL64, r
 
 
 def test_device_argument_errors(hp):
