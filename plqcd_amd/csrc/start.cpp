@@ -71,14 +71,14 @@ extern "C" int init_plqcd(int argc, char *argv[])
 {
    plqcd::clear_error();
    if (argc < 2 || argv == nullptr || argv[1] == nullptr) {
-      printf("Error: must pass input file name for the global paramters to be read from");
+      printf("init_plqcd: the first argument must name the input file\n");
       return plqcd::set_error(-1, "init_plqcd: argv[1] must name the input file");
    }
    if (plqcd_g.init == 1) return 0;
 
    FILE *fp = fopen(argv[1], "r");
    if (!fp) {
-      fprintf(stderr, "Error opening input file. \n");
+      fprintf(stderr, "init_plqcd: cannot open input file '%s'\n", argv[1]);
       return plqcd::set_error(1, "init_plqcd: cannot open input file '%s'", argv[1]);
    }
    set_defaults();
@@ -138,7 +138,7 @@ extern "C" void comp_cpr(int ip, int n[])
    const int *p = plqcd_g.nprocs;
    const int np = p[0] * p[1] * p[2] * p[3];
    if (ip < 0 || ip >= np) {
-      printf("Error in comp_cpr, the process id must be  0 =< ip < %d.\n", np);
+      printf("comp_cpr: process id %d is outside 0 .. %d\n", ip, np - 1);
       plqcd::set_error(-1, "comp_cpr: process id %d out of range", ip);
       return;
    }
@@ -166,7 +166,7 @@ extern "C" void lex2cart(int ix, int xcart[])
 {
    const int *L = plqcd_g.latdims;
    if (ix < 0 || ix >= plqcd_g.VOLUME) {
-      printf("Error lex2cart expects the lexiographic index to be  0 =< ix < VOLUME.\n");
+      printf("lex2cart: index %d is outside 0 .. VOLUME-1\n", ix);
       plqcd::set_error(-1, "lex2cart: index %d out of range", ix);
       return;
    }
@@ -181,8 +181,7 @@ extern "C" int cart2lex(int xcor[])
    const int *L = plqcd_g.latdims;
    for (int i = 0; i < 4; i++)
       if (xcor[i] < 0 || xcor[i] >= L[i]) {
-         printf("Error in converting cartesian coordinates of a site to lexiographic index:"
-                "coordiante in the %d th direction is out of bounds\n", i);
+         printf("cart2lex: coordinate %d = %d is outside the local lattice\n", i, xcor[i]);
          return plqcd::set_error(-1, "cart2lex: coordinate %d = %d out of range", i, xcor[i]);
       }
    return xcor[0] + L[0] * (xcor[1] + L[1] * (xcor[2] + L[2] * xcor[3]));
@@ -256,7 +255,7 @@ extern "C" void find_site(int xg[], int *pid, int *xeo)
    int proc[4], xc[4];
    for (int mu = 0; mu < 4; mu++) {
       if (xg[mu] < 0 || xg[mu] >= plqcd_g.nprocs[mu] * plqcd_g.latdims[mu]) {
-         printf("Error, global site coordinates are out of bounds in the %d -th direction\n", mu);
+         printf("find_site: global coordinate %d = %d is outside the lattice\n", mu, xg[mu]);
          plqcd::set_error(-1, "find_site: coordinate %d = %d out of range", mu, xg[mu]);
          return;
       }
