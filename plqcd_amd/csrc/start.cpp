@@ -31,8 +31,7 @@ int check_params()
    const int g[4] = {plqcd_g.LX, plqcd_g.LY, plqcd_g.LZ, plqcd_g.LT};
    for (int mu = 0; mu < 4; mu++)
       if (g[mu] < 4 || (g[mu] % 2) != 0)
-         return plqcd::set_error(2, "global lattice dimension in each direction must be >= 4 and even "
-                                    "(L%d = %d)", mu, g[mu]);
+         return plqcd::set_error(2, "lattice extent L%d = %d: every extent must be even and at least 4", mu, g[mu]);
    for (int mu = 0; mu < 4; mu++)
       if (plqcd_g.nprocs[mu] != 1)
          return plqcd::set_error(3, "NPROC%d = %d: this library runs one process on one GPU, "

@@ -36,7 +36,7 @@ extern "C" void *alloc(size_t size, size_t alignment)
    void *ptr = nullptr;
    if (posix_memalign(&ptr, alignment, size) != 0 || ptr == nullptr) {
       plqcd::set_error(1, "alloc(): posix_memalign of %zu bytes failed", size);
-      fprintf(stderr, " alloc() returned NULL. Out of memory?\n");
+      fprintf(stderr, "alloc(): could not allocate %zu bytes\n", size);
       return nullptr;
    }
    return ptr;
