@@ -15,6 +15,7 @@
 #include <cstdint>
 #include <cstring>
 #include <mutex>
+#include <new>
 #include <thread>
 #include <vector>
 
@@ -42,10 +43,10 @@ class CopyPool {
    // variable until the process ends (joining them from a static destructor
    // would run while other threads may still copy, and a forked child holds
    // thread objects of threads it does not have)
-   static CopyPool &get()
+   static CopyPool *get()
    {
-      static CopyPool *pool = new CopyPool();
-      return *pool;
+      static CopyPool *pool = new (std::nothrow) CopyPool();
+      return pool;  // null if out of memory: the caller copies on its own thread
    }
 
    void copy(void *dst, const void *src, size_t n)
@@ -78,8 +79,17 @@ class CopyPool {
    CopyPool()
    {
       owner_ = getpid();
-      nworkers_ = pool_threads() - 1;
-      for (int k = 0; k < nworkers_; k++) th_.emplace_back([this, k] { worker(k + 1); });
+      const int want = pool_threads() - 1;
+      // a thread that cannot be started leaves a smaller pool (no workers:
+      // plain memcpy); nothing may throw through the C ABI
+      for (int k = 0; k < want; k++) {
+         try {
+            th_.emplace_back([this, k] { worker(k + 1); });
+         } catch (...) {
+            break;
+         }
+         nworkers_ = k + 1;
+      }
    }
    void run_piece(int k)
    {
@@ -121,5 +131,8 @@ class CopyPool {
 
 void plqcd::host_copy(void *dst, const void *src, size_t n)
 {
-   CopyPool::get().copy(dst, src, n);
+   if (CopyPool *pool = CopyPool::get())
+      pool->copy(dst, src, n);
+   else
+      std::memcpy(dst, src, n);
 }
