@@ -656,7 +656,21 @@ extern "C" int plqcd_hip_server_create(const plqcd_serve_config *cfg, plqcd_hip_
    plqcd_hip_server *s = new (std::nothrow) plqcd_hip_server();
    if (!s) return plqcd::set_error(4, "out of host memory");
    s->cfg = *cfg;
-   PLQCD_HIP_TRY(hipSetDevice(cfg->device));
+   // every failure below stops the threads already started and frees s;
+   // nothing may throw through the C ABI
+   auto fail = [s](int rc) {
+      {
+         std::lock_guard<std::mutex> lk(s->xmu);
+         s->xstop = true;
+         s->xcv.notify_all();
+      }
+      for (std::thread &t : s->xfer)
+         if (t.joinable()) t.join();
+      delete s;
+      return rc;
+   };
+   if (hipError_t e = hipSetDevice(cfg->device); e != hipSuccess)
+      return fail(plqcd::set_error(10, "server: hipSetDevice(%d): %s", cfg->device, hipGetErrorString(e)));
    // one stream per engine and per transfer thread: with HIP's default of 4
    // hardware queues per process unrelated streams share queues and wait for
    // each other (a 60 s request stream took 53.6 s of GPU work instead of
@@ -671,14 +685,19 @@ extern "C" int plqcd_hip_server_create(const plqcd_serve_config *cfg, plqcd_hip_
                       nq);
    }
    size_t free_b = 0, total_b = 0;
-   PLQCD_HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+   if (hipError_t e = hipMemGetInfo(&free_b, &total_b); e != hipSuccess)
+      return fail(plqcd::set_error(10, "server: hipMemGetInfo: %s", hipGetErrorString(e)));
    const size_t margin = (size_t)8 << 30;  // the runtime's own allocations, transfer stagers, fragmentation
    s->budget = cfg->hbm_budget ? cfg->hbm_budget : (free_b > margin ? free_b - margin : free_b / 2);
    if (s->budget > free_b) s->budget = free_b;
    s->t0 = clock_t_::now();
    const int nx = cfg->transfer_threads > 0 ? cfg->transfer_threads : kTransferThreads;
-   for (int i = 0; i < nx; i++) s->xfer.emplace_back(xfer_main, s);
-   s->worker = std::thread(worker_main, s);
+   try {
+      for (int i = 0; i < nx; i++) s->xfer.emplace_back(xfer_main, s);
+      s->worker = std::thread(worker_main, s);
+   } catch (...) {
+      return fail(plqcd::set_error(4, "server: cannot start its threads"));
+   }
    *out = s;
    return 0;
 }
