@@ -51,9 +51,13 @@ void parallel_sites(int V, F f)
    for (unsigned t = 0; t < nt; t++) {
       const int lo = t * chunk, hi = std::min(V, lo + chunk);
       if (lo >= hi) break;
-      pool.emplace_back([=] {
+      try {
+         pool.emplace_back([=] {
+            for (int i = lo; i < hi; i++) f(i);
+         });
+      } catch (...) {  // no thread: the chunk runs here (each site's value depends on its index only)
          for (int i = lo; i < hi; i++) f(i);
-      });
+      }
    }
    for (auto &th : pool) th.join();
 }
