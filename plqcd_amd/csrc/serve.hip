@@ -669,8 +669,10 @@ extern "C" int plqcd_hip_server_create(const plqcd_serve_config *cfg, plqcd_hip_
       delete s;
       return rc;
    };
-   if (hipError_t e = hipSetDevice(cfg->device); e != hipSuccess)
+   if (hipError_t e = hipSetDevice(cfg->device); e != hipSuccess) {
+      (void)hipGetLastError();  // not left for a later, unrelated check
       return fail(plqcd::set_error(10, "server: hipSetDevice(%d): %s", cfg->device, hipGetErrorString(e)));
+   }
    // one stream per engine and per transfer thread: with HIP's default of 4
    // hardware queues per process unrelated streams share queues and wait for
    // each other (a 60 s request stream took 53.6 s of GPU work instead of

@@ -99,3 +99,16 @@ def test_pinned_buffers_move_by_dma(hp):
     with pytest.raises(ValueError):
         with Server(slots=1) as s:
             s.submit(L, 5, b, 0.123, 1e-9, out=np.empty((1,) + b.shape[1:], np.complex128))
+
+
+def test_server_create_rejects_bad_config(hp):
+    """Creation errors come back as error codes: a device that does not exist
+    (hipSetDevice fails after the server object exists) and bad slot or
+    thread counts; a good server is created afterwards."""
+    from plqcd_amd import capi
+    from plqcd_amd.serve import Server
+    for kw in ({"device": 1000}, {"slots": 0}, {"slots": 33}, {"transfer_threads": 65}):
+        with pytest.raises(capi.PlqcdError):
+            Server(**kw)
+    with Server(slots=2) as s:
+        assert s is not None
