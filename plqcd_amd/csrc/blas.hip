@@ -264,7 +264,7 @@ int blas_cg_step(const plqcd_hip_field *r, const double *beta, plqcd_hip_field *
 {
    if (int rc = check_pair(r, p, w) | check_pair(r, x, w)) return rc;
    if (r->prec != p->prec || r->prec != x->prec) return set_error(1, "blas_cg_step: precision mismatch");
-   if (p == x || p == r) return set_error(1, "blas_cg_step: p must be distinct from r and x");
+   if (p == x || p == r || r == x) return set_error(1, "blas_cg_step: r, p and x must be distinct fields");
    const BlasArgs a = make_args(r, nullptr, w);
    hipStream_t st = r->lat->stream;
    return dispatch1(r->prec, [&](auto P) {
