@@ -15,6 +15,7 @@ boundaries, Eq. 2.1 (wilson mode).  Prints one JSON line.
   formats   = the same H in every storage format: FP64 12-real links, FP32
               and FP16 with 18/12-real links, FP64 18-real (200 launches each)
   compat    = the reference's own operator (compat mode), device-resident
+  schur     = the even-odd operator 1 - kappa^2 H_eo H_oe on the even sites
   dropin    = wall time per call of the drop-in eo then oe on host arrays
               (transfers included), links uploaded per call or cached, and
               of the fused call
@@ -214,6 +215,34 @@ def compat_run(L, steps=200, warmup=10):
     t = ms / 1e3 / steps
     return {"us_per_launch": round(t * 1e6, 2), "gflops": round(FLOP_PER_SITE * V / t / 1e9, 1),
             "frac_of_hbm_peak": round(960 * V / t / 1e9 / HBM_PEAK_GBS, 4)}
+
+
+def schur_run(L, steps=100, warmup=10):
+    """The even-odd (Schur) operator out_e = in_e - kappa^2 H_eo H_oe in_e
+    (plqcd_hip_apply_schur: two half-lattice hops, each reading every link,
+    then one vector pass on the even sites; each call synchronises), FP64
+    18-real, one RHS, kappa 0.124.  Compulsory bytes per application, per site
+    of the lattice: 2 x (576 link + 96 spinor in + 96 spinor out) for the hops
+    and 3 x 96 for the vector pass = 1824 B."""
+    from plqcd_amd import hopping as hp, solver as sv
+    V = int(np.prod(L))
+    lat = hp.Lattice(L)
+    g = hp.Gauge(lat).upload(hp.random_gauge_su3(V, 1))
+    fi = hp.Field(lat).upload(hp.random_spinor(V, 2))
+    tmp, fo = hp.Field(lat), hp.Field(lat)
+    for _ in range(warmup):
+        sv.apply_schur(g, fi, tmp, fo, 0.124, 0)
+    lat.timer_start()
+    for _ in range(steps):
+        sv.apply_schur(g, fi, tmp, fo, 0.124, 0)
+    t = lat.timer_stop() / 1e3 / steps  # seconds per application
+    for f in (fi, tmp, fo, g):
+        f.close()
+    lat.close()
+    nbytes = 1824 * V
+    return {"us_per_application": round(t * 1e6, 1), "compulsory_bytes_per_site": 1824,
+            "compulsory_GBps": round(nbytes / t / 1e9, 1), "frac_of_hbm_peak": round(nbytes / t / 1e9 / HBM_PEAK_GBS, 4),
+            "parity": "even", "steps": steps}
 
 
 def dropin_run(L, calls=5):
@@ -723,6 +752,7 @@ def main():
     cpu = None if args.no_cpu else cpu_baseline(L, args.cpu_seconds)
     case = HopCase(L, prec, links, mode)
     compat = compat_run(L) if headline and not args.no_formats else None
+    schur = schur_run(L) if headline and not args.no_formats else None
     dropin = dropin_run(L) if headline and not args.no_formats else None
     solver = None if args.no_solver else solve_compare(L)
     cg48 = cg48_compare() if headline and not args.no_solver else None
@@ -769,6 +799,7 @@ def main():
         "cpu_baseline": cpu,
         "formats": formats,
         "compat": compat,
+        "schur": schur,
         "dropin": dropin,
         "solver": solver,
         "cg_48c96": cg48,
