@@ -219,11 +219,11 @@ def compat_run(L, steps=200, warmup=10):
 
 def schur_run(L, steps=100, warmup=10):
     """The even-odd (Schur) operator out_e = in_e - kappa^2 H_eo H_oe in_e
-    (plqcd_hip_apply_schur: two half-lattice hops, each reading every link,
-    then one vector pass on the even sites; each call synchronises), FP64
-    18-real, one RHS, kappa 0.124.  Compulsory bytes per application, per site
-    of the lattice: 2 x (576 link + 96 spinor in + 96 spinor out) for the hops
-    and 3 x 96 for the vector pass = 1824 B."""
+    (plqcd_hip_apply_schur: two half-lattice hops, each reading every link, the
+    second forming in_e - kappa^2 (H tmp) in its epilogue; each call
+    synchronises), FP64 18-real, one RHS, kappa 0.124.  Compulsory bytes per
+    application, per site of the lattice: 2 x (576 link + 96 spinor in + 96
+    spinor out) for the hops and 96 for the centre in_e = 1632 B."""
     from plqcd_amd import hopping as hp, solver as sv
     V = int(np.prod(L))
     lat = hp.Lattice(L)
@@ -239,8 +239,8 @@ def schur_run(L, steps=100, warmup=10):
     for f in (fi, tmp, fo, g):
         f.close()
     lat.close()
-    nbytes = 1824 * V
-    return {"us_per_application": round(t * 1e6, 1), "compulsory_bytes_per_site": 1824,
+    nbytes = 1632 * V
+    return {"us_per_application": round(t * 1e6, 1), "compulsory_bytes_per_site": 1632,
             "compulsory_GBps": round(nbytes / t / 1e9, 1), "frac_of_hbm_peak": round(nbytes / t / 1e9 / HBM_PEAK_GBS, 4),
             "parity": "even", "steps": steps}
 

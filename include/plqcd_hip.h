@@ -232,7 +232,8 @@ int plqcd_hip_apply_m_noalloc(const plqcd_hip_gauge *g, const plqcd_hip_field *i
 
 /* Even-odd (Schur) preconditioned operator on one parity p (EVEN or ODD):
  * out_p = in_p - kappa_r^2 H_{p,q} H_{q,p} in_p  (q the other parity), built
- * from the eo/oe halves of dirac.h:21-22 in wilson mode.  tmp receives
+ * from the eo/oe halves of dirac.h:21-22 in wilson mode (the second half
+ * hop forms in_p - kappa^2 (H tmp) in its epilogue).  tmp receives
  * H_{q,p} in_p on parity q; the other parity of out is not touched.
  * Synchronous. */
 int plqcd_hip_apply_schur(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip_field *tmp,

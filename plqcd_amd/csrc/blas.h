@@ -30,8 +30,6 @@ int blas_accum(plqcd_hip_field *y, plqcd_hip_field *x, int zero_x, const int *ma
 // y = x ; partial <- |y|^2   (any precisions)
 int blas_copy(plqcd_hip_field *y, const plqcd_hip_field *x, double *partial, const int *mask,
               const Window &w = Window{});
-// y = x - c[r] y on the sites of `parity` only
-int blas_xmcy_parity(const plqcd_hip_field *x, plqcd_hip_field *y, const double *c, int parity);
 // y = 0 for RHS r with mask[r] != 0 (every RHS when mask is null)
 int blas_zero(plqcd_hip_field *y, const int *mask, const Window &w = Window{});
 

@@ -138,23 +138,6 @@ __global__ __launch_bounds__(kBlock) void k_zero(SpinorIO<P> yv, const int *mask
    store_d(yv.at_rhs(r), p, j, z);
 }
 
-// y = x - c[r] * y on the sites of one parity (the other parity untouched)
-template <int P>
-__global__ __launch_bounds__(kBlock) void k_xmcy_parity(SpinorIO<P> xv, SpinorIO<P> yv, const double *c, int parity,
-                                                        BlasArgs a)
-{
-   int p, j, js, r;
-   if (!blas_site(a, p, j, js, r) || p != parity) return;
-   const double cc = c[r];
-   const SpinorIO<P> X = xv.at_rhs(r), Y = yv.at_rhs(r);
-   d2 xs[12], ys[12];
-   load_d(X, p, j, xs);
-   load_d(Y, p, j, ys);
-#pragma unroll
-   for (int m = 0; m < 12; m++) ys[m] = d2{fma(-cc, ys[m].x, xs[m].x), fma(-cc, ys[m].y, xs[m].y)};
-   store_d(Y, p, j, ys);
-}
-
 // out[r] = sum of partial[r*stride + first + k], k < n, in an order fixed by n
 // alone.  Stage 1: block (b, r) sums segment b of kSeg values (thread t adds
 // elements t, t+256, ... of the segment, then a block tree) and stores the sum
@@ -307,19 +290,6 @@ int blas_copy(plqcd_hip_field *y, const plqcd_hip_field *x, double *partial, con
    return dispatch2(y->prec, x->prec, [&](auto PY, auto PX) {
       constexpr int A = decltype(PY)::value, B = decltype(PX)::value;
       hipLaunchKernelGGL((k_copy<A, B>), grid_of(a), block_of(a), 0, st, spinor_io<A>(y), spinor_io<B>(x), mask, a);
-   });
-}
-
-int blas_xmcy_parity(const plqcd_hip_field *x, plqcd_hip_field *y, const double *c, int parity)
-{
-   if (int rc = check_pair(x, y, Window{})) return rc;
-   if (x->prec != y->prec) return set_error(1, "blas_xmcy_parity: precision mismatch");
-   const BlasArgs a = make_args(y, nullptr, Window{});
-   hipStream_t st = y->lat->stream;
-   return dispatch1(y->prec, [&](auto P) {
-      constexpr int PP = decltype(P)::value;
-      hipLaunchKernelGGL(k_xmcy_parity<PP>, grid_of(a), block_of(a), 0, st, spinor_io<PP>(x), spinor_io<PP>(y), c,
-                         parity, a);
    });
 }
 

@@ -53,6 +53,9 @@ namespace plqcd {
 //   alpha   != null: CG residual update (with dag, kappa and partial): Ap = in -
 //                    kappa H^+ in is not stored; out -= alpha[r] Ap (out holds
 //                    the residual), whose |out|^2 goes to partial
+//   centre  != null: (with kappa; no dag, norm, alpha or resid_b) out = centre -
+//                    kappa[r] * (H in) on the output parity, the centre read
+//                    from that field rather than from in (gather kernel)
 //   resid_b != null: residual (with dag, kappa and partial; F64 storage):
 //                    out = resid_b - (in - kappa H^+ in) for every RHS of the
 //                    launch, stored in out's own precision (any), and its |out|^2
@@ -68,6 +71,7 @@ struct DslashOp {
    const double *kappa = nullptr;
    double *partial = nullptr;
    const double *alpha = nullptr;
+   const plqcd_hip_field *centre = nullptr;
    const plqcd_hip_field *resid_b = nullptr;
 };
 

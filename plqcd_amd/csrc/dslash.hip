@@ -158,6 +158,13 @@ int plqcd::dslash(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip
       return plqcd::set_error(1, "hop: RHS list of %d entries for fields of %d", op.nr, in->nrhs);
    if (!op.rlist && (op.nr < 0 || op.r0 < 0 || (op.nr > 0 && op.r0 + op.nr > in->nrhs)))
       return plqcd::set_error(1, "hop: RHS window %d + %d outside the fields' %d", op.r0, op.nr, in->nrhs);
+   if (op.centre) {
+      const plqcd_hip_field *c = op.centre;
+      if (c->lat != g->lat || c->prec != g->prec || c->nrhs != in->nrhs || c == out || !op.kappa || op.dag ||
+          op.alpha || op.partial || op.resid_b)
+         return plqcd::set_error(1, "hop: the centre field must be like in, distinct from out, with kappa and no "
+                                    "other epilogue");
+   }
    if (op.resid_b) {
       const plqcd_hip_field *b = op.resid_b;
       if (g->prec != PLQCD_PREC_F64 || b->lat != g->lat || b->prec != PLQCD_PREC_F64 || b->nrhs != in->nrhs ||

@@ -419,11 +419,11 @@ __device__ __forceinline__ int rhs_of(const HopArgs &a, int r)
    return a.rlist ? a.rlist[r] : a.r0 + r;
 }
 
-// Operands of the epilogues: the per-RHS step alpha (EPI = 2) and the
-// right-hand side b (EPI = 3).
+// Operands of the epilogues: the per-RHS step alpha (EPI = 2), the
+// right-hand side b (EPI = 3) and the centre field (EPI = 4).
 template <class SIO>
 struct CgFuse {
-   SIO p;  // EPI 3: b
+   SIO p;  // EPI 3: b; EPI 4: the centre field
    const double *alpha;
 };
 
@@ -440,6 +440,9 @@ struct CgFuse {
 // RHS being updated).  No per-RHS predicate: a data-dependent condition
 // around the stores made the compiler keep both paths' values live (280
 // registers, one wave per SIMD, against 144 and three waves).
+// EPI = 4 is EPI = 1 with the centre taken from another field, f.p: out =
+// c(x) - kappa_r * (H in)(x) (the even-odd operator's second half hop, whose
+// input lives on the other parity; gather kernel only).
 // With NORM the site's |out|^2 joins the deterministic chunk reduction.  `in`
 // and `out` are positioned at the RHS `r`, whose kappa and alpha apply.
 template <int EPI, bool NORM, class SIO, class SO>
@@ -450,7 +453,10 @@ __device__ __forceinline__ double epilogue(const SIO &in, const SO &out, const H
    typedef typename SIO::C C;
    if (EPI >= 1) {
       C c0[12];
-      if (cen == nullptr) in.load(p, j, c0);
+      if constexpr (EPI == 4)
+         f.p.at_rhs(r).load(p, j, c0);
+      else if (cen == nullptr)
+         in.load(p, j, c0);
       const real kap = (real)a.kappa[r];
 #pragma unroll
       for (int m = 0; m < 12; m++) acc[m] = cxmky(cen ? cen[m] : c0[m], kap, acc[m]);
@@ -1096,6 +1102,13 @@ int launch_one(int variant, dim3 grid, dim3 block, hipStream_t st, S si, SO so, 
                const CgFuse<S> &f)
 {
    constexpr bool same = std::is_same<S, SO>::value && EPI != 3;  // the tile and RHS-blocked kernels
+   if constexpr (EPI == 4) {  // the external-centre epilogue exists for the gather kernel only
+      if (variant == V_GATHER) {
+         hipLaunchKernelGGL((hop_kernel<MODE, DAG, EPI, NORM, BLOCK, S, G, SO>), grid, block, 0, st, si, so, gg, a, f);
+         return 0;
+      }
+      return plqcd::set_error(2, "hop: the external-centre epilogue runs the gather kernel only (variant %d)", variant);
+   }
    if (variant == V_MARCH) {
       hipLaunchKernelGGL((hop_tmarch_kernel<MODE, DAG, EPI, NORM, BLOCK, S, G, SO>), grid, block, 0, st, si, so, gg, a,
                          f);
@@ -1145,6 +1158,7 @@ int launch_t(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip_fiel
    const S si = spinor_io<PREC>(in);
    const G gg = gauge_io<PREC, LINKS>(g);
    const bool resid = op.resid_b != nullptr;  // EPI 3: out may have another precision
+   const bool centre = op.centre != nullptr;  // EPI 4: the gather kernel only
    // lanes: the window of right-hand sides [op.r0, op.r0 + R) of the fields
    const int R = op.nr > 0 ? op.nr : in->nrhs;
    a.r0 = op.nr > 0 && !op.rlist ? op.r0 : 0;
@@ -1154,9 +1168,9 @@ int launch_t(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip_fiel
    plqcd::lookup_tuning(plqcd::TuneKey{{lat->L[0], lat->L[1], lat->L[2], lat->L[3]}, PREC, LINKS, R,
                                        solver ? plqcd::kTuneOpSolver : plqcd::kTuneOpHop},
                         &tu);
-   // RHS per lane (RHS-blocked gather) when it divides R
-   // (the residual epilogue runs the gather kernel only)
-   const int rb = !resid && (tu.rb == 2 || tu.rb == 4) && R % tu.rb == 0 ? tu.rb : 1;
+   // RHS per lane (RHS-blocked gather) when it divides R; none for the
+   // residual epilogue, and the external-centre one runs the gather kernel only
+   const int rb = !resid && !centre && (tu.rb == 2 || tu.rb == 4) && R % tu.rb == 0 ? tu.rb : 1;
    const int lanes = R / rb;  // lanes per site
    // block shape: `lanes` lanes per site and a whole number of reduction chunks
    int target = tu.block;
@@ -1174,17 +1188,21 @@ int launch_t(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip_fiel
    a.partial = op.partial;
    a.nchunk = plqcd::reduce_chunks(lat->Vh);
    const bool dag = op.dag != 0;
-   const int epi = resid ? 3 : op.alpha ? 2 : op.kappa ? 1 : 0;
+   const int epi = resid ? 3 : op.alpha ? 2 : op.centre ? 4 : op.kappa ? 1 : 0;
    const bool norm = op.partial != nullptr;
    if ((dag || epi || norm) && mode != PLQCD_HOP_WILSON)
       return plqcd::set_error(2, "hop: M, M^+ and norms need the wilson mode");
    if (epi == 2 && (!dag || !norm || !op.kappa))
       return plqcd::set_error(2, "hop: the CG-update epilogue needs M^+, a norm and kappa");
+   if (epi == 4 && (dag || norm || !op.kappa))
+      return plqcd::set_error(2, "hop: the external-centre epilogue needs kappa, and no M^+ or norm");
    if (epi == 3 && (PREC != PLQCD_PREC_F64 || !dag || !norm || !op.kappa))
       return plqcd::set_error(2, "hop: the residual epilogue needs F64 storage, M^+, a norm and kappa");
    CgFuse<S> fu{};
    if (epi == 2) {
       fu.alpha = op.alpha;
+   } else if (epi == 4) {
+      fu.p = spinor_io<PREC>(op.centre);
    } else if (epi == 3) {
       fu.p = spinor_io<PREC>(op.resid_b);
    }
@@ -1193,7 +1211,8 @@ int launch_t(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip_fiel
    // the t-march kernel serves full-lattice applications (either mode) whose
    // time extent is a multiple of T (and, with a norm, whose t-slices hold
    // whole reduction chunks)
-   const bool march = rb == 1 && T > 0 && a.nparity == 2 && lat->L[3] % T == 0 && (!norm || a.S % REDUCE_CHUNK == 0);
+   const bool march = !centre && rb == 1 && T > 0 && a.nparity == 2 && lat->L[3] % T == 0 &&
+                      (!norm || a.S % REDUCE_CHUNK == 0);
    // the LDS-tiled t-march: one RHS, a tile of whole x rows dividing the y and
    // z extents, 2 Lxh ty tz lanes per block (at most 256 with F64 storage for
    // the LDS), and, with a norm, reduction chunks that do not leave a tile row
@@ -1242,6 +1261,8 @@ int launch_t(const plqcd_hip_gauge *g, const plqcd_hip_field *in, plqcd_hip_fiel
       lrc = launch_block<PLQCD_HOP_COMPAT, false, 0, false>(mblock, variant, grid, block, st, si, so, gg, a);
    } else if (!dag) {
       if (epi == 0) lrc = launch_block<PLQCD_HOP_WILSON, false, 0, false>(mblock, variant, grid, block, st, si, so, gg, a);
+      else if (epi == 4)
+         lrc = launch_block<PLQCD_HOP_WILSON, false, 4, false>(mblock, variant, grid, block, st, si, so, gg, a, fu);
       else if (!norm) lrc = launch_block<PLQCD_HOP_WILSON, false, 1, false>(mblock, variant, grid, block, st, si, so, gg, a);
       else lrc = launch_block<PLQCD_HOP_WILSON, false, 1, true>(mblock, variant, grid, block, st, si, so, gg, a);
    } else {

@@ -562,10 +562,13 @@ extern "C" int plqcd_hip_apply_schur(const plqcd_hip_gauge *g, const plqcd_hip_f
    hipError_t e = hipMemcpyAsync(dk, k2.data(), R * sizeof(double), hipMemcpyHostToDevice, lat->stream);
    int rc = e == hipSuccess ? 0 : plqcd::set_error(10, "apply_schur: %s", hipGetErrorString(e));
    const int other = parity == PLQCD_PARITY_EVEN ? PLQCD_PARITY_ODD : PLQCD_PARITY_EVEN;
-   // tmp(other) = H in(parity) ; out(parity) = H tmp(other) ; out = in - kappa^2 out on parity
+   // tmp(other) = H in(parity) ; out(parity) = in(parity) - kappa^2 H tmp(other),
+   // the second half hop with the centre taken from in
+   plqcd::DslashOp m;
+   m.kappa = dk;
+   m.centre = in;
    if (!rc) rc = plqcd_hip_hop(g, in, tmp, other, PLQCD_HOP_WILSON);
-   if (!rc) rc = plqcd_hip_hop(g, tmp, out, parity, PLQCD_HOP_WILSON);
-   if (!rc) rc = plqcd::blas_xmcy_parity(in, out, dk, parity);
+   if (!rc) rc = plqcd::dslash(g, tmp, out, parity, PLQCD_HOP_WILSON, m);
    (void)hipFreeAsync(dk, lat->stream);
    if (!rc) PLQCD_HIP_TRY(hipStreamSynchronize(lat->stream));
    return rc;
