@@ -212,3 +212,24 @@ def test_schur_operator_matches_oracle_composition(hp, parity):
         got = out.download(r)
         assert site_rel_err(got[half], ref[half]) < 1e-13
         assert np.array_equal(got[other], sentinel[other])
+
+
+@pytest.mark.parametrize("prec,links,tol", [(1, 18, 1e-5), (1, 12, 1e-5), (2, 12, 1e-2)])
+def test_schur_operator_reduced_precision(hp, prec, links, tol):
+    """The Schur operator with FP32 / FP16 storage (the external-centre
+    epilogue in float) against the FP64 oracle composition, per-site relative
+    error within the storage's tolerance (two hops, an FP32 or FP16 tmp)."""
+    from plqcd_amd import solver as sv
+    from oracle import MODE_WILSON, hopping_eo, hopping_oe
+    L = (4, 6, 4, 8)
+    V = int(np.prod(L))
+    u = hp.random_gauge_su3(V, 61)
+    q = hp.random_spinor(V, 62)
+    lat = hp.Lattice(L)
+    g = hp.Gauge(lat, prec, links).upload(u)
+    fi = hp.Field(lat, prec).upload(q)
+    tmp, out = hp.Field(lat, prec), hp.Field(lat, prec)
+    sv.apply_schur(g, fi, tmp, out, 0.124, 0)
+    ref = q - 0.124 ** 2 * hopping_eo(hopping_oe(q, u, L, MODE_WILSON), u, L, MODE_WILSON)
+    err = site_rel_err(out.download()[: V // 2], ref[: V // 2])
+    assert err < tol, err
